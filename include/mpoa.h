@@ -1,0 +1,115 @@
+/* mpoa: MI355X (gfx950) partial-order alignment, C ABI.
+ *
+ * Drop-in for minipoa's (NCl3-lhd/POA) alignment path. Every entry point names the upstream
+ * interface it replaces. Plain pointers and sizes only; the library owns its device memory
+ * through an mpoa_ctx, callers own every host buffer they pass.
+ *
+ * Scores are minipoa's int32 arithmetic (src/align.cpp:21-22); results are bit-identical to
+ * minipoa on the same inputs and flags.
+ */
+#ifndef MPOA_H
+#define MPOA_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* minipoa's flags (src/main.cpp:14-35, struct para_t in include/parameter.h:5-19), taken
+ * as typed on the command line and normalised as initPara does
+ * (src/handle_input.cpp:45-77). */
+typedef struct mpoa_params {
+  int32_t match, mismatch, gap_open, gap_ext; /* -M -X -O -E (defaults 2 -4 -4 -2) */
+  int32_t band_b, band_f;                     /* -b -f (defaults 100 40) */
+  int32_t ab_band;                            /* -B */
+  int32_t seeding;                            /* -S */
+  int32_t k, mm_w;                            /* -k -w (defaults 19 10) */
+  int32_t poa_w;                              /* -W; 0 = flag not given */
+  int32_t result;                             /* -r: 0 consensus, 1 row-column MSA, 2 GFA */
+} mpoa_params;
+
+/* Error codes. minipoa terminates the process in these cases; the library returns. */
+enum {
+  MPOA_OK = 0,
+  MPOA_E_ARG = -1,
+  MPOA_E_BAND = -2,         /* backtrack left the band (minipoa: exit(0), align.cpp:1584) */
+  MPOA_E_BACKTRACK = -3,    /* no move found (minipoa: "backtrack error", exit(1), :1730) */
+  MPOA_E_OUT_CAP = -4,      /* caller buffer too small; the size needed is reported */
+  MPOA_E_EMPTY_WINDOW = -5, /* -S sub-window ends before it begins (minipoa: SIGSEGV) */
+  MPOA_E_OOM = -6,          /* device allocation failed (minipoa: throw, mem_alloc_utils.cpp:27) */
+  MPOA_E_DEVICE = -7,       /* HIP runtime error, or no GPU */
+  MPOA_E_OVERFLOW = -8      /* internal: a window outgrew its pre-sized arena; the library
+                               re-runs it with exact sizes and never returns this */
+};
+
+typedef struct mpoa_ctx mpoa_ctx;
+
+/* Opens the library on HIP device `device`. Fails with MPOA_E_DEVICE when no GPU is present:
+ * there is no CPU fallback. */
+int mpoa_create(int device, mpoa_ctx** out);
+void mpoa_destroy(mpoa_ctx* ctx);
+const char* mpoa_strerror(int rc);
+
+/* Statistics of one DP call (one upstream poa() invocation). */
+typedef struct mpoa_call_stat {
+  int32_t read;         /* read index in input order */
+  int32_t rows;         /* n = end_rank - beg_rank + 1 */
+  int32_t qlen;
+  int32_t score;        /* H[n-1][m]: what minipoa prints as "score:" under -V 1 (align.cpp:1577) */
+  int64_t cells;        /* sum over rows 1..n-1 of 8*block_num (align.cpp:1446) */
+  int64_t edge_cells;   /* sum over rows of in-window in-degree * 8*block_num */
+} mpoa_call_stat;
+
+/* minipoa's whole main loop over one read set: for each read in input order,
+ * alignment() -> graph::add_path -> graph::topsort (src/main.cpp:110-121), then the output
+ * selected by params->result (main.cpp:124-126: graph::output_consensus, output_rc_msa,
+ * output_gfa). Replaces the loop in src/main.cpp and its calls into src/align.cpp and
+ * src/graph.cpp. The graph and the DP live on the device.
+ *
+ * Reads: seq with n_reads+1 offsets. name/comment (optional, NULL) are laid out the same way
+ * and only used by the MSA/GFA headers. Output text (exactly minipoa's stdout) goes to
+ * out[0..out_cap); *out_len receives its size (also when MPOA_E_OUT_CAP is returned).
+ * stats (optional) receives one record per DP call. */
+int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, const char* seq,
+                     const int64_t* seq_off, const char* name, const int64_t* name_off,
+                     const char* comment, const int64_t* comment_off, char* out,
+                     int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
+                     int32_t stats_cap, int32_t* n_stats);
+
+/* ------------------------------------------------------------------------------------
+ * Batched windows. minipoa has no batch interface: it aligns one read set per process
+ * (src/main.cpp:110-121). A batch is many independent read sets ("windows"), each the
+ * equivalent of one minipoa run, all processed by one persistent device launch.
+ * ------------------------------------------------------------------------------------ */
+typedef struct mpoa_batch mpoa_batch;
+
+typedef struct mpoa_batch_stats {
+  int64_t windows;       /* windows in the batch */
+  int64_t dp_calls;      /* DP calls made (reads after the first of each window) */
+  int64_t cells;         /* sum of 8*block_num over all DP rows */
+  int64_t edge_cells;
+  int64_t reruns;        /* windows re-run after an arena overflow */
+  double device_ms;      /* device time of the last execute, HIP events */
+} mpoa_batch_stats;
+
+/* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window
+ * arenas and allocates device memory. size_hint (optional, n_windows entries, may be NULL)
+ * are predicted DP cells per window, used to order the work queue and to pre-size the DP
+ * arenas; any misprediction is caught on the device and that window re-run exactly. */
+int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windows,
+                       const int32_t* win_off, const char* seq, const int64_t* seq_off,
+                       const double* size_hint, mpoa_batch** out);
+/* Runs the whole per-window pipeline on the device for every window; inputs are already
+ * resident. Synchronous. */
+int mpoa_batch_execute(mpoa_batch* b);
+/* Copies the per-window output text (minipoa's stdout for that window) to out; window w is
+ * out[out_off[w] .. out_off[w+1]). */
+int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_off,
+                       mpoa_batch_stats* stats);
+void mpoa_batch_free(mpoa_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

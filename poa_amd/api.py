@@ -1,0 +1,210 @@
+"""Python binding of the C ABI in include/mpoa.h (ctypes over poa_amd/libmpoa.so).
+
+Mirrors minipoa's interface: ``Params`` carries minipoa's flags (src/main.cpp:14-35);
+``align_reads`` is minipoa's main loop (src/main.cpp:84-126) and returns exactly the text
+minipoa prints; ``Batch`` runs many independent windows in one device launch.
+
+There is no CPU path: every call fails with ``MpoaError`` when the library or the GPU is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import PRODUCT_SO
+
+MPOA_OK = 0
+ERRORS = {
+    -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
+    -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW",
+}
+
+EXPORTS = [
+    "mpoa_create", "mpoa_destroy", "mpoa_strerror", "mpoa_align_reads", "mpoa_batch_prepare",
+    "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free",
+]
+
+
+class MpoaError(RuntimeError):
+    def __init__(self, rc: int, msg: str):
+        super().__init__(f"mpoa error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+        self.rc = rc
+
+
+class Params(ctypes.Structure):
+    """minipoa flags; defaults are minipoa's (src/main.cpp:18-33)."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "match", "mismatch", "gap_open", "gap_ext", "band_b", "band_f", "ab_band", "seeding",
+        "k", "mm_w", "poa_w", "result")]
+
+    def __init__(self, match=2, mismatch=-4, gap_open=-4, gap_ext=-2, band_b=100, band_f=40,
+                 ab_band=0, seeding=0, k=19, mm_w=10, poa_w=0, result=0):
+        super().__init__(match, mismatch, gap_open, gap_ext, band_b, band_f, ab_band, seeding,
+                         k, mm_w, poa_w, result)
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class CallStat(ctypes.Structure):
+    _fields_ = [("read", ctypes.c_int32), ("rows", ctypes.c_int32), ("qlen", ctypes.c_int32),
+                ("score", ctypes.c_int32), ("cells", ctypes.c_int64),
+                ("edge_cells", ctypes.c_int64)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("windows", ctypes.c_int64), ("dp_calls", ctypes.c_int64),
+                ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
+                ("reruns", ctypes.c_int64), ("device_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(PRODUCT_SO):
+            raise MpoaError(-7, f"{PRODUCT_SO} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(PRODUCT_SO)
+        P, I32, I64, VP = ctypes.POINTER, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+        L.mpoa_create.argtypes = [ctypes.c_int, P(VP)]
+        L.mpoa_destroy.argtypes = [VP]
+        L.mpoa_destroy.restype = None
+        L.mpoa_strerror.argtypes = [ctypes.c_int]
+        L.mpoa_strerror.restype = ctypes.c_char_p
+        L.mpoa_align_reads.argtypes = [VP, P(Params), I32, VP, VP, VP, VP, VP, VP, VP, I64,
+                                       P(I64), VP, I32, P(I32)]
+        L.mpoa_batch_prepare.argtypes = [VP, P(Params), I32, VP, VP, VP, VP, P(VP)]
+        L.mpoa_batch_execute.argtypes = [VP]
+        L.mpoa_batch_results.argtypes = [VP, VP, I64, VP, P(BatchStats)]
+        L.mpoa_batch_free.argtypes = [VP]
+        L.mpoa_batch_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != MPOA_OK:
+        raise MpoaError(rc, lib().mpoa_strerror(rc).decode())
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _names(names: list[str] | None):
+    if not names:
+        return None, None
+    enc = [n.encode() for n in names]
+    off = np.zeros(len(enc) + 1, np.int64)
+    off[1:] = np.cumsum([len(e) for e in enc])
+    buf = np.frombuffer(b"".join(enc) + b"\0", np.uint8).copy()
+    return buf, off
+
+
+class Context:
+    """Device context (mpoa_create). One per process and device is enough."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().mpoa_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().mpoa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def align_reads(self, params: Params, seq: np.ndarray, off: np.ndarray,
+                    names: list[str] | None = None, comments: list[str] | None = None,
+                    with_stats: bool = False):
+        """minipoa's main loop on the GPU. Returns the stdout text (and DP call stats)."""
+        n = len(off) - 1
+        seq = np.ascontiguousarray(seq, np.uint8)
+        off = np.ascontiguousarray(off, np.int64)
+        nb, no = _names(names)
+        cb, co = _names(comments)
+        need = ctypes.c_int64()
+        stats = (CallStat * max(n, 1))()
+        n_stats = ctypes.c_int32()
+        rc = lib().mpoa_align_reads(self.h, ctypes.byref(params), n, _ptr(seq), _ptr(off),
+                                    _ptr(nb), _ptr(no), _ptr(cb), _ptr(co), None, 0,
+                                    ctypes.byref(need), stats, max(n, 1), ctypes.byref(n_stats))
+        _check(rc)
+        buf = np.zeros(max(need.value, 1), np.uint8)
+        rc = lib().mpoa_align_reads(self.h, ctypes.byref(params), n, _ptr(seq), _ptr(off),
+                                    _ptr(nb), _ptr(no), _ptr(cb), _ptr(co), _ptr(buf),
+                                    len(buf), ctypes.byref(need), None, 0, None)
+        _check(rc)
+        text = buf[:need.value].tobytes().decode()
+        if with_stats:
+            return text, [stats[i] for i in range(n_stats.value)]
+        return text
+
+
+@dataclass
+class BatchResult:
+    texts: list[str]
+    stats: BatchStats
+    device_ms: float = field(default=0.0)
+
+
+class Batch:
+    """Independent windows prepared on the device (mpoa_batch_prepare)."""
+
+    def __init__(self, ctx: Context, params: Params, win_off: np.ndarray, seq: np.ndarray,
+                 seq_off: np.ndarray, size_hint: np.ndarray | None = None):
+        self.ctx = ctx
+        self.win_off = np.ascontiguousarray(win_off, np.int32)
+        seq = np.ascontiguousarray(seq, np.uint8)
+        seq_off = np.ascontiguousarray(seq_off, np.int64)
+        hint = None if size_hint is None else np.ascontiguousarray(size_hint, np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().mpoa_batch_prepare(ctx.h, ctypes.byref(params), len(self.win_off) - 1,
+                                        _ptr(self.win_off), _ptr(seq), _ptr(seq_off), _ptr(hint),
+                                        ctypes.byref(h)))
+        self.h = h
+        self.n = len(self.win_off) - 1
+
+    def execute(self) -> float:
+        """Runs every window on the device; returns device milliseconds."""
+        _check(lib().mpoa_batch_execute(self.h))
+        return self.stats().device_ms
+
+    def stats(self) -> BatchStats:
+        st = BatchStats()
+        _check(lib().mpoa_batch_results(self.h, None, 0, None, ctypes.byref(st)))
+        return st
+
+    def results(self) -> BatchResult:
+        off = np.zeros(self.n + 1, np.int64)
+        st = BatchStats()
+        _check(lib().mpoa_batch_results(self.h, None, 0, _ptr(off), ctypes.byref(st)))
+        buf = np.zeros(max(int(off[-1]), 1), np.uint8)
+        _check(lib().mpoa_batch_results(self.h, _ptr(buf), len(buf), _ptr(off),
+                                        ctypes.byref(st)))
+        raw = buf.tobytes()
+        texts = [raw[off[i]:off[i + 1]].decode() for i in range(self.n)]
+        return BatchResult(texts, st, st.device_ms)
+
+    def free(self) -> None:
+        if self.h:
+            lib().mpoa_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

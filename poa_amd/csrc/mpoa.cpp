@@ -1,0 +1,552 @@
+// Host side of the C ABI (include/mpoa.h): device context, batch sizing, launches of the
+// device pipeline (pipeline.hip), overflow re-runs and output formatting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpoa.h"
+#include "pipeline.h"
+
+using mpoa::CallStat;
+using mpoa::DpScores;
+using mpoa::KernelArgs;
+using mpoa::SlotCaps;
+using mpoa::WinDesc;
+using mpoa::WinResult;
+
+struct mpoa_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  size_t mem_total = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
+      return e_ == hipErrorOutOfMemory ? MPOA_E_OOM : MPOA_E_DEVICE;                      \
+    }                                                                                    \
+  } while (0)
+
+// initPara (src/handle_input.cpp:45-77) + o1 = O + E (src/align.cpp:1369).
+DpScores scores_of(const mpoa_params& p) {
+  DpScores s;
+  s.match = p.match;
+  s.mismatch = p.mismatch > 0 ? -p.mismatch : p.mismatch;
+  const int O = p.gap_open > 0 ? -p.gap_open : p.gap_open;
+  s.e1 = p.gap_ext > 0 ? -p.gap_ext : p.gap_ext;
+  s.o1 = O + s.e1;
+  s.b = p.band_b;
+  s.f = p.band_f;
+  s.ab_band = p.ab_band;
+  s.result = p.result;
+  return s;
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      return MPOA_E_OOM;
+    }
+    n = count;
+    return MPOA_OK;
+  }
+};
+
+struct WinInfo {
+  int n_reads;
+  int qmax;
+  int64_t qtot;
+};
+
+// Capacities estimated from the reads alone. Nodes: the longest read plus a quarter of the
+// other bases (final graphs of 5-15% error reads hold ~0.1 node per input base); DP cells:
+// nodes times the static band width 2w + 64. The device checks every capacity exactly.
+struct Estimate {
+  int nodes, edges;
+  int64_t dp;
+};
+
+Estimate estimate(const WinInfo& w, const DpScores& s, bool full_width) {
+  Estimate e;
+  int64_t n = 2 + w.qmax + (int64_t)std::ceil(0.25 * (double)(w.qtot - w.qmax));
+  n = std::min<int64_t>(n, 2 + w.qtot);
+  e.nodes = (int)n;
+  e.edges = (int)std::min<int64_t>(3 * n + w.n_reads, w.qtot + w.n_reads + 2);
+  const int m = w.qmax + 1;
+  int64_t width;
+  if (s.f > 0 && !full_width) {
+    const int wb = s.b + m / s.f;
+    width = std::min<int64_t>(2LL * wb + 64, (int64_t)(m / 8 + 2) * 8);
+  } else {
+    width = (int64_t)(m / 8 + 2) * 8;
+  }
+  e.dp = n * width;
+  return e;
+}
+
+// Exact worst case: every base a new node; every row full width.
+Estimate worst_case(const WinInfo& w) {
+  Estimate e;
+  e.nodes = (int)std::min<int64_t>(2 + w.qtot, INT32_MAX / 64);
+  e.edges = (int)std::min<int64_t>(w.qtot + w.n_reads + 2, INT32_MAX / 64);
+  e.dp = (int64_t)e.nodes * ((w.qmax + 1) / 8 + 2) * 8;
+  return e;
+}
+
+}  // namespace
+
+struct mpoa_batch {
+  mpoa_ctx* ctx = nullptr;
+  mpoa_params params{};
+  DpScores sc{};
+  int n_windows = 0;
+  std::vector<WinInfo> info;
+  std::vector<WinDesc> desc;
+  std::vector<int> order;     // processing order (largest predicted first)
+  std::vector<double> cost;   // predicted cost per window
+  int64_t out_total = 0;
+  DevBuf<uint8_t> d_seq;
+  DevBuf<int64_t> d_off;
+  DevBuf<WinDesc> d_win;
+  DevBuf<WinResult> d_res;
+  DevBuf<int> d_order;
+  DevBuf<int> d_queue;
+  DevBuf<uint8_t> d_out;
+  DevBuf<int> d_int;
+  DevBuf<uint8_t> d_byte;
+  DevBuf<int> d_dp;
+  SlotCaps caps{};
+  int n_slots = 0;
+  int nt = 64, cpl = 4;
+  std::vector<WinResult> res;
+  int64_t reruns = 0;
+  double device_ms = 0;
+  // per-call statistics (single-window runs)
+  DevBuf<CallStat> d_calls;
+  int calls_stride = 0;
+};
+
+namespace {
+
+SlotCaps make_caps(const std::vector<Estimate>& est, const std::vector<WinInfo>& info,
+                   const std::vector<int>& wins, bool keep_path) {
+  SlotCaps c{};
+  c.nodes = 2;
+  c.edges = 4;
+  c.qmax = 1;
+  c.path = 0;
+  c.dp = 64;
+  for (int w : wins) {
+    c.nodes = std::max(c.nodes, est[w].nodes);
+    c.edges = std::max(c.edges, est[w].edges);
+    c.qmax = std::max(c.qmax, info[w].qmax);
+    if (keep_path) c.path = (int)std::max<int64_t>(c.path, info[w].qtot);
+    c.dp = std::max(c.dp, est[w].dp);
+  }
+  c.nodes = (c.nodes + 63) / 64 * 64;
+  c.edges = (c.edges + 63) / 64 * 64;
+  c.dp = (c.dp + 63) / 64 * 64;
+  c.ints_per_slot = (mpoa::slot_ints(c.nodes, c.edges, c.qmax, c.path) + 63) / 64 * 64;
+  c.bytes_per_slot = (mpoa::slot_bytes(c.nodes, c.edges, c.qmax) + 255) / 256 * 256;
+  return c;
+}
+
+size_t slot_bytes_total(const SlotCaps& c) {
+  return (size_t)c.ints_per_slot * 4 + (size_t)c.bytes_per_slot + (size_t)c.dp * 12;
+}
+
+// One launch of the device pipeline over `wins` with capacities `caps`.
+int launch(mpoa_batch* b, const std::vector<int>& wins, const SlotCaps& caps, int max_slots) {
+  mpoa_ctx* ctx = b->ctx;
+  const size_t per_slot = slot_bytes_total(caps);
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  // Keep a quarter of free memory back for the runtime and other users.
+  size_t budget = free_b / 4 * 3;
+  int n_slots = (int)std::min<size_t>({(size_t)max_slots, wins.size(), budget / per_slot});
+  if (n_slots < 1) {
+    g_err = "window too large for device memory";
+    return MPOA_E_OOM;
+  }
+  int rc;
+  if ((rc = b->d_int.alloc((size_t)n_slots * caps.ints_per_slot)) != MPOA_OK) return rc;
+  if ((rc = b->d_byte.alloc((size_t)n_slots * caps.bytes_per_slot)) != MPOA_OK) return rc;
+  if ((rc = b->d_dp.alloc((size_t)n_slots * 3 * caps.dp)) != MPOA_OK) return rc;
+  HIPCHK(hipMemcpyAsync(b->d_order.p, wins.data(), wins.size() * sizeof(int),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(b->d_queue.p, 0, sizeof(int), ctx->stream));
+  KernelArgs a{};
+  a.scores = b->sc;
+  a.caps = caps;
+  a.read_seq = b->d_seq.p;
+  a.read_off = b->d_off.p;
+  a.win = b->d_win.p;
+  a.result = b->d_res.p;
+  a.order = b->d_order.p;
+  a.n_jobs = (int)wins.size();
+  a.queue = b->d_queue.p;
+  a.int_pool = b->d_int.p;
+  a.byte_pool = b->d_byte.p;
+  a.dp_pool = b->d_dp.p;
+  a.out = b->d_out.p;
+  a.call_stats = b->d_calls.p;
+  a.call_stats_stride = b->calls_stride;
+  a.call_stats_cap = b->calls_stride;
+  const int lrc = mpoa_launch_windows(a, n_slots, b->nt, b->cpl, ctx->stream);
+  if (lrc != 0) {
+    g_err = lrc == -2 ? "unsupported tile shape" : "kernel launch failed";
+    return lrc == -2 ? MPOA_E_ARG : MPOA_E_DEVICE;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  b->n_slots = n_slots;
+  b->caps = caps;
+  return MPOA_OK;
+}
+
+int upload(mpoa_batch* b, int32_t n_windows, const int32_t* win_off, const char* seq,
+           const int64_t* seq_off) {
+  mpoa_ctx* ctx = b->ctx;
+  const int32_t r0 = win_off[0], r1 = win_off[n_windows];
+  const int64_t base0 = seq_off[r0], bases = seq_off[r1] - base0;
+  std::vector<int64_t> off(r1 - r0 + 1);
+  for (int32_t r = r0; r <= r1; r++) off[r - r0] = seq_off[r] - base0;
+  int rc;
+  if ((rc = b->d_seq.alloc((size_t)bases + 16)) != MPOA_OK) return rc;
+  if ((rc = b->d_off.alloc(off.size())) != MPOA_OK) return rc;
+  HIPCHK(hipMemcpyAsync(b->d_seq.p, seq + base0, (size_t)bases, hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(b->d_off.p, off.data(), off.size() * sizeof(int64_t),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MPOA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpoa_strerror(int rc) {
+  switch (rc) {
+    case MPOA_OK: return "ok";
+    case MPOA_E_ARG: return g_err.empty() ? "invalid argument" : g_err.c_str();
+    case MPOA_E_BAND: return "backtrack left the band";
+    case MPOA_E_BACKTRACK: return "backtrack error";
+    case MPOA_E_OUT_CAP: return "output buffer too small";
+    case MPOA_E_EMPTY_WINDOW: return "empty seeding sub-window";
+    case MPOA_E_OOM: return g_err.empty() ? "device out of memory" : g_err.c_str();
+    case MPOA_E_DEVICE: return g_err.empty() ? "HIP device error" : g_err.c_str();
+    case MPOA_E_OVERFLOW: return "arena overflow";
+    default: return "unknown error";
+  }
+}
+
+int mpoa_create(int device, mpoa_ctx** out) {
+  if (!out) return MPOA_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= device || device < 0) {
+    g_err = "no HIP device available (the library has no CPU path)";
+    return MPOA_E_DEVICE;
+  }
+  HIPCHK(hipSetDevice(device));
+  mpoa_ctx* c = new mpoa_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    g_err = "device initialisation failed";
+    return MPOA_E_DEVICE;
+  }
+  c->n_cu = prop.multiProcessorCount;
+  c->mem_total = prop.totalGlobalMem;
+  *out = c;
+  return MPOA_OK;
+}
+
+void mpoa_destroy(mpoa_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windows,
+                       const int32_t* win_off, const char* seq, const int64_t* seq_off,
+                       const double* size_hint, mpoa_batch** out) {
+  if (!ctx || !params || !out || n_windows < 0 || !win_off || !seq_off || (!seq && n_windows))
+    return MPOA_E_ARG;
+  *out = nullptr;
+  if (params->seeding) {
+    g_err = "seeding (-S) windows are not supported by the batch pipeline";
+    return MPOA_E_ARG;
+  }
+  if (params->result == 2) {
+    g_err = "GFA output is not supported by the batch pipeline";
+    return MPOA_E_ARG;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  mpoa_batch* b = new mpoa_batch();
+  b->ctx = ctx;
+  b->params = *params;
+  b->sc = scores_of(*params);
+  b->n_windows = n_windows;
+  b->info.resize(n_windows);
+  b->desc.resize(n_windows);
+  b->cost.resize(n_windows);
+  const int r_base = win_off[0];
+  int64_t out0 = 0;
+  for (int w = 0; w < n_windows; w++) {
+    WinInfo& wi = b->info[w];
+    wi.n_reads = win_off[w + 1] - win_off[w];
+    wi.qmax = 0;
+    wi.qtot = 0;
+    for (int r = win_off[w]; r < win_off[w + 1]; r++) {
+      const int64_t l = seq_off[r + 1] - seq_off[r];
+      wi.qmax = (int)std::max<int64_t>(wi.qmax, l);
+      wi.qtot += l;
+    }
+    WinDesc& d = b->desc[w];
+    d.read0 = win_off[w] - r_base;
+    d.n_reads = wi.n_reads;
+    d.out0 = out0;
+    int64_t cap;
+    if (params->result == 1) cap = (int64_t)wi.n_reads * (2 + wi.qtot);
+    else cap = 2 * (int64_t)wi.qmax + 64;
+    cap = std::min<int64_t>(cap, INT32_MAX / 2);
+    d.out_cap = (int)cap;
+    out0 += (cap + 15) / 16 * 16;
+    const Estimate e = estimate(wi, b->sc, params->ab_band != 0 && params->band_f > 0);
+    b->cost[w] = size_hint ? size_hint[w] : (double)wi.n_reads * (double)e.dp;
+  }
+  b->out_total = out0;
+  b->order.resize(n_windows);
+  for (int w = 0; w < n_windows; w++) b->order[w] = w;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return b->cost[x] > b->cost[y]; });
+  int rc;
+  if (n_windows > 0 && (rc = upload(b, n_windows, win_off, seq, seq_off)) != MPOA_OK) {
+    delete b;
+    return rc;
+  }
+  if ((rc = b->d_win.alloc(std::max(n_windows, 1))) != MPOA_OK ||
+      (rc = b->d_res.alloc(std::max(n_windows, 1))) != MPOA_OK ||
+      (rc = b->d_order.alloc(std::max(n_windows, 1))) != MPOA_OK ||
+      (rc = b->d_queue.alloc(1)) != MPOA_OK ||
+      (rc = b->d_out.alloc((size_t)std::max<int64_t>(out0, 16))) != MPOA_OK) {
+    delete b;
+    return rc;
+  }
+  if (n_windows > 0 &&
+      hipMemcpy(b->d_win.p, b->desc.data(), n_windows * sizeof(WinDesc), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    delete b;
+    return MPOA_E_DEVICE;
+  }
+  *out = b;
+  return MPOA_OK;
+}
+
+int mpoa_batch_execute(mpoa_batch* b) {
+  if (!b) return MPOA_E_ARG;
+  mpoa_ctx* ctx = b->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = b->n_windows;
+  b->res.assign(n, WinResult{});
+  b->reruns = 0;
+  if (n == 0) return MPOA_OK;
+  const bool keep_path = b->params.result != 0;
+  const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
+  std::vector<Estimate> est(n);
+  for (int w = 0; w < n; w++) est[w] = estimate(b->info[w], b->sc, full);
+
+  hipEvent_t ev0, ev1;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventRecord(ev0, ctx->stream));
+  // Pass 1: every window with estimated capacities (largest estimate sets the slot size).
+  const int max_slots = std::max(1, ctx->n_cu * (b->nt == 64 ? 16 : 4));
+  std::vector<int> wins = b->order;
+  int rc = launch(b, wins, make_caps(est, b->info, wins, keep_path), max_slots);
+  if (rc != MPOA_OK) return rc;
+  std::vector<WinResult> res(n);
+  HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+  // Later passes: windows that overflowed, with worst-case capacities.
+  for (int pass = 0; pass < 2; pass++) {
+    std::vector<int> again;
+    for (int w : wins)
+      if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
+    if (again.empty()) break;
+    b->reruns += (int64_t)again.size();
+    for (int w : again) est[w] = worst_case(b->info[w]);
+    rc = launch(b, again, make_caps(est, b->info, again, keep_path), max_slots);
+    if (rc != MPOA_OK) return rc;
+    HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+    wins = again;
+  }
+  HIPCHK(hipEventRecord(ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  b->device_ms = ms;
+  b->res = res;
+  for (int w = 0; w < n; w++)
+    if (res[w].status != MPOA_OK) return res[w].status;
+  return MPOA_OK;
+}
+
+int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_off,
+                       mpoa_batch_stats* stats) {
+  if (!b) return MPOA_E_ARG;
+  const int n = b->n_windows;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->windows = n;
+    for (int w = 0; w < n && w < (int)b->res.size(); w++) {
+      stats->cells += b->res[w].cells;
+      stats->edge_cells += b->res[w].edge_cells;
+      stats->dp_calls += std::max(0, b->info[w].n_reads - 1);
+    }
+    stats->reruns = b->reruns;
+    stats->device_ms = b->device_ms;
+  }
+  if (!out_off) return MPOA_OK;
+  std::vector<uint8_t> raw((size_t)std::max<int64_t>(b->out_total, 1));
+  if (b->out_total > 0)
+    HIPCHK(hipMemcpy(raw.data(), b->d_out.p, (size_t)b->out_total, hipMemcpyDeviceToHost));
+  static const char kHead[] = ">consensus sequence\n";
+  int64_t at = 0;
+  out_off[0] = 0;
+  for (int w = 0; w < n; w++) {
+    const WinResult& r = b->res[w];
+    const uint8_t* src = raw.data() + b->desc[w].out0;
+    std::string text;
+    if (b->params.result == 0) {
+      text.assign(kHead);
+      text.append((const char*)src, (size_t)std::max(0, r.out_len));
+      text.push_back('\n');
+    } else {
+      const int L = std::max(0, r.out_len);
+      for (int i = 0; i < b->info[w].n_reads; i++) {
+        text += ">" + std::to_string(i + 1) + "\n";
+        text.append((const char*)src + (int64_t)i * L, (size_t)L);
+        text.push_back('\n');
+      }
+    }
+    if (out && at + (int64_t)text.size() <= out_cap) std::memcpy(out + at, text.data(), text.size());
+    at += (int64_t)text.size();
+    out_off[w + 1] = at;
+  }
+  return out && at > out_cap ? MPOA_E_OUT_CAP : MPOA_OK;
+}
+
+void mpoa_batch_free(mpoa_batch* b) { delete b; }
+
+int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, const char* seq,
+                     const int64_t* seq_off, const char* name, const int64_t* name_off,
+                     const char* comment, const int64_t* comment_off, char* out,
+                     int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
+                     int32_t stats_cap, int32_t* n_stats) {
+  if (!ctx || !params || n_reads < 0 || (n_reads > 0 && (!seq || !seq_off))) return MPOA_E_ARG;
+  int32_t win_off[2] = {0, n_reads};
+  std::vector<int64_t> zero_off{0};
+  const int64_t* so = n_reads > 0 ? seq_off : zero_off.data();
+  mpoa_batch* b = nullptr;
+  int rc = mpoa_batch_prepare(ctx, params, 1, win_off, seq ? seq : "", so, nullptr, &b);
+  if (rc != MPOA_OK) return rc;
+  b->nt = 256;
+  b->cpl = 2;
+  b->calls_stride = std::max(1, n_reads);
+  if ((rc = b->d_calls.alloc((size_t)b->calls_stride)) != MPOA_OK) {
+    mpoa_batch_free(b);
+    return rc;
+  }
+  rc = mpoa_batch_execute(b);
+  const WinResult r = b->res.empty() ? WinResult{} : b->res[0];
+  if (n_stats) *n_stats = std::max(0, n_reads - 1);
+  if (stats && n_reads > 1) {
+    std::vector<CallStat> cs(n_reads - 1);
+    if (hipMemcpy(cs.data(), b->d_calls.p, cs.size() * sizeof(CallStat), hipMemcpyDeviceToHost) ==
+        hipSuccess) {
+      for (int i = 0; i < (int)cs.size() && i < stats_cap; i++) {
+        stats[i].read = i + 1;
+        stats[i].rows = cs[i].rows;
+        stats[i].qlen = cs[i].qlen;
+        stats[i].score = cs[i].score;
+        stats[i].cells = 0;
+        stats[i].edge_cells = 0;
+      }
+    }
+  }
+  if (rc != MPOA_OK) {
+    mpoa_batch_free(b);
+    return rc;
+  }
+  std::string text;
+  if (params->result == 0) {
+    std::vector<char> cons((size_t)std::max(r.out_len, 1));
+    if (r.out_len > 0 &&
+        hipMemcpy(cons.data(), b->d_out.p, (size_t)r.out_len, hipMemcpyDeviceToHost) != hipSuccess) {
+      mpoa_batch_free(b);
+      return MPOA_E_DEVICE;
+    }
+    text = ">consensus sequence\n";
+    text.append(cons.data(), (size_t)r.out_len);
+    text.push_back('\n');
+  } else {
+    // output_rc_msa (src/graph.cpp:237-251)
+    const int L = r.out_len;
+    std::vector<char> rows((size_t)std::max<int64_t>((int64_t)n_reads * L, 1));
+    if ((int64_t)n_reads * L > 0 &&
+        hipMemcpy(rows.data(), b->d_out.p, (size_t)n_reads * L, hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+      mpoa_batch_free(b);
+      return MPOA_E_DEVICE;
+    }
+    for (int i = 0; i < n_reads; i++) {
+      std::string nm, cm;
+      if (name && name_off) nm.assign(name + name_off[i], name + name_off[i + 1]);
+      if (comment && comment_off) cm.assign(comment + comment_off[i], comment + comment_off[i + 1]);
+      if (nm.empty()) text += ">" + std::to_string(i + 1) + "\n";
+      else text += ">" + nm + " " + cm + "\n";
+      text.append(rows.data() + (int64_t)i * L, (size_t)L);
+      text.push_back('\n');
+    }
+  }
+  mpoa_batch_free(b);
+  if (out_len) *out_len = (int64_t)text.size();
+  if (!out) return MPOA_OK;
+  if ((int64_t)text.size() > out_cap) return MPOA_E_OUT_CAP;
+  std::memcpy(out, text.data(), text.size());
+  return MPOA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// Kernel-argument structures shared by the device pipeline (pipeline.hip) and the host
+// library (mpoa.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpoa.h"
+
+namespace mpoa {
+
+// Scores after minipoa's normalisation: o1 = gap_open + gap_ext (src/align.cpp:1369).
+struct DpScores {
+  int match, mismatch, o1, e1;
+  int b, f;
+  int ab_band;
+  int result;
+};
+
+// Capacities of one slot (one resident workgroup). A window that outgrows any of them
+// stops with MPOA_E_OVERFLOW and is re-run by the host with larger capacities.
+struct SlotCaps {
+  int nodes;       // graph nodes
+  int edges;       // edge records per direction
+  int qmax;        // longest read
+  int path;        // sum of read lengths kept for MSA/GFA paths (0 in consensus mode)
+  int64_t dp;      // int32 cells per DP matrix (H, D and I each)
+  int64_t ints_per_slot;
+  int64_t bytes_per_slot;
+};
+
+// Int32 and byte words a slot's arena takes; must match carve() in pipeline.hip.
+inline __host__ __device__ int64_t slot_ints(int N, int E, int qmax, int path) {
+  return 31LL * N + 8LL * E + 2LL * qmax + 8 + path;
+}
+inline __host__ __device__ int64_t slot_bytes(int N, int E, int qmax) {
+  return (int64_t)N + E + 2LL * qmax + 64;
+}
+
+struct WinDesc {
+  int read0, n_reads;  // reads [read0, read0 + n_reads) of the read table
+  int64_t out0;        // output bytes [out0, out0 + out_cap)
+  int out_cap;
+  int pad;
+};
+
+struct WinResult {
+  int status;
+  int out_len;     // consensus length, or MSA columns
+  int64_t cells, edge_cells;
+  int nodes, edges;
+};
+
+struct CallStat {
+  int score, rows, qlen, pad;
+};
+
+struct KernelArgs {
+  DpScores scores;
+  SlotCaps caps;
+  const uint8_t* read_seq;
+  const int64_t* read_off;
+  const WinDesc* win;
+  WinResult* result;
+  const int* order;
+  int n_jobs;
+  int* queue;
+  int* int_pool;
+  uint8_t* byte_pool;
+  int* dp_pool;
+  uint8_t* out;
+  CallStat* call_stats;     // optional: per window, per DP call
+  int call_stats_stride;
+  int call_stats_cap;
+};
+
+}  // namespace mpoa
+
+// Launches the persistent pipeline with n_slots workgroups of nt threads, cpl DP columns per
+// thread per pass. Returns 0, or -2 for an unsupported (nt, cpl).
+int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
+                        hipStream_t stream);

@@ -1,0 +1,109 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so), the checker the GPU path is compared
+against. Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg import this module."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "minipoa")
+ORACLE_CLI = os.path.join(ROOT, "oracle", "oracle_cli")
+
+
+class OParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "match", "mismatch", "gap_open", "gap_ext", "band_b", "band_f", "ab_band", "seeding",
+        "k", "mm_w", "poa_w", "result")]
+
+
+class OCallStat(ctypes.Structure):
+    _fields_ = [("read", ctypes.c_int32), ("rows", ctypes.c_int32), ("qlen", ctypes.c_int32),
+                ("score", ctypes.c_int32), ("ab_band", ctypes.c_int32),
+                ("max_width", ctypes.c_int32), ("cells", ctypes.c_int64),
+                ("edge_cells", ctypes.c_int64), ("logical_cells", ctypes.c_int64),
+                ("graph_nodes", ctypes.c_int32), ("graph_edges", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} not built (make -C oracle)")
+        _lib = ctypes.CDLL(ORACLE_SO)
+    return _lib
+
+
+def oparams(p) -> OParams:
+    """From poa_amd.api.Params (or a dict) to the oracle's struct."""
+    d = p.as_dict() if hasattr(p, "as_dict") else dict(p)
+    base = dict(match=2, mismatch=-4, gap_open=-4, gap_ext=-2, band_b=100, band_f=40,
+                ab_band=0, seeding=0, k=19, mm_w=10, poa_w=0, result=0)
+    base.update(d)
+    return OParams(**base)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _names(names):
+    if not names:
+        return None, None
+    enc = [n.encode() for n in names]
+    off = np.zeros(len(enc) + 1, np.int64)
+    off[1:] = np.cumsum([len(e) for e in enc])
+    return np.frombuffer(b"".join(enc) + b"\0", np.uint8).copy(), off
+
+
+def window(p, seq: np.ndarray, off: np.ndarray, names=None, comments=None):
+    """minipoa's stdout for one read set, plus per-poa()-call statistics."""
+    L = lib()
+    seq = np.ascontiguousarray(seq, np.uint8)
+    off = np.ascontiguousarray(off, np.int64)
+    n = len(off) - 1
+    op = oparams(p)
+    nb, no = _names(names)
+    cb, co = _names(comments)
+    need = ctypes.c_int64()
+    cap = 1 << 16
+    stats = (OCallStat * cap)()
+    ns = ctypes.c_int32()
+    rc = L.orc_window(ctypes.byref(op), n, _p(seq), _p(off), _p(nb), _p(no), _p(cb), _p(co),
+                      None, ctypes.c_int64(0), ctypes.byref(need), stats, cap, ctypes.byref(ns))
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    buf = np.zeros(max(need.value, 1), np.uint8)
+    rc = L.orc_window(ctypes.byref(op), n, _p(seq), _p(off), _p(nb), _p(no), _p(cb), _p(co),
+                      _p(buf), ctypes.c_int64(len(buf)), ctypes.byref(need), None, 0, None)
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    return buf[:need.value].tobytes().decode(), [stats[i] for i in range(ns.value)]
+
+
+def batch(p, win_off, seq, seq_off, threads=0):
+    """Independent windows on `threads` OpenMP threads; list of stdout texts."""
+    L = lib()
+    win_off = np.ascontiguousarray(win_off, np.int32)
+    seq = np.ascontiguousarray(seq, np.uint8)
+    seq_off = np.ascontiguousarray(seq_off, np.int64)
+    nw = len(win_off) - 1
+    op = oparams(p)
+    out_off = np.zeros(nw + 1, np.int64)
+    rc = L.orc_batch(ctypes.byref(op), nw, _p(win_off), _p(seq), _p(seq_off), threads, None,
+                     ctypes.c_int64(0), _p(out_off), None, ctypes.c_int64(0), None)
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    buf = np.zeros(max(int(out_off[-1]), 1), np.uint8)
+    rc = L.orc_batch(ctypes.byref(op), nw, _p(win_off), _p(seq), _p(seq_off), threads, _p(buf),
+                     ctypes.c_int64(len(buf)), _p(out_off), None, ctypes.c_int64(0), None)
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    raw = buf.tobytes()
+    return [raw[out_off[i]:out_off[i + 1]].decode() for i in range(nw)]
