@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: batched uneven POA windows on one MI355X, bit-exact to minipoa.
+
+Workload (default): 10,000 independent POA jobs, each 5-60 reads over a template whose length
+is lognormal (median 500 bp, sigma 0.5, clipped to 150-2,000 bp), per-job error uniform in
+5-15% (sub:ins:del 1:1:1), minipoa's default scoring and band (-M2 -X-4 -O-4 -E-2 -b100
+-f40), consensus output. A step is one pass of the whole per-window pipeline (DP, backtrack,
+add-to-graph, re-sort, consensus) over all windows, inputs resident in HBM.
+
+Prints one JSON line: windows/s (value), GCUPS, the dominant kernel's roofline figures and the
+upstream minipoa binary timed on the host cores over a bounded sample of the same windows.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "minipoa")
+
+
+def cpu_baseline(batch, gpu_texts, sample: int, cores: int) -> dict:
+    """Upstream minipoa (oracle/_ref/minipoa, built from its own sources) on `sample` windows
+    spread over the batch, `cores` processes at a time; the CPU oracle if the binary is absent.
+    Also checks the GPU outputs of those windows against the CPU outputs."""
+    n = batch.n_windows
+    idx = np.linspace(0, n - 1, num=min(sample, n)).round().astype(int)
+    idx = np.unique(idx)
+    if os.path.exists(REF_BIN):
+        tmp = tempfile.mkdtemp(prefix="mpoa_cpu_")
+        try:
+            paths = []
+            for w in idx:
+                pth = os.path.join(tmp, f"w{w}.fa")
+                batch.window(int(w)).to_fasta(pth)
+                paths.append(pth)
+
+            def run(pth):
+                return subprocess.run([REF_BIN, pth], check=True, capture_output=True).stdout
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as ex:
+                outs = list(ex.map(run, paths))
+            wall = time.perf_counter() - t0
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        texts = [o.decode() for o in outs]
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        from poa_amd import api
+        sub = batch.subset(idx)
+        t0 = time.perf_counter()
+        texts = oracle_lib.batch(api.Params(), sub.win_off, sub.seq, sub.seq_off, threads=cores)
+        wall = time.perf_counter() - t0
+        kind = "port"
+    same = all(texts[i] == gpu_texts[int(w)] for i, w in enumerate(idx))
+    return {
+        "value": round(len(idx) / wall, 2),
+        "unit": "windows/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"{len(idx)} of the {n} windows (evenly spaced), one minipoa process per "
+                  f"window, {cores} at a time; wall {wall:.2f} s",
+        "outputs_identical_to_gpu": bool(same),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--windows", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--nt", type=int, default=None, help="threads per window workgroup")
+    ap.add_argument("--cpl", type=int, default=None, help="DP columns per thread")
+    ap.add_argument("--cpu-sample", type=int, default=320)
+    ap.add_argument("--cpu-cores", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(windows are independent: N GPUs would each run a 1/N share of them).",
+              file=sys.stderr)
+        return 2
+
+    from poa_amd import api, synth
+
+    seed = synth.SEEDS[3] if args.seed is None else args.seed
+    batch = synth.config3(seed=seed, n_windows=args.windows)
+    ctx = api.Context(0)
+    params = api.Params()
+    b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
+    if args.nt or args.cpl:
+        b.set_tile(args.nt or 64, args.cpl or 4)
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    log(f"{args.windows} windows, {int(batch.win_off[-1])} reads prepared")
+    for i in range(args.warmup):
+        log(f"warmup {i}: {b.execute():.1f} ms")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        b.execute()
+        kernel_ms.append(b.stats().kernel_ms)
+        log(f"step {i}: kernel {kernel_ms[-1]:.1f} ms")
+    wall = time.perf_counter() - t0
+    res = b.results()
+    st = res.stats
+    ws, tick_s = b.window_stats()
+    ph = np.array([[w.ticks[k] for k in range(6)] for w in ws], dtype=np.float64) * tick_s
+    slow = int(np.argmax(ph.sum(axis=1)))
+    log("phase seconds summed over windows: " +
+        ", ".join(f"{n}={v:.3f}" for n, v in zip(api.PHASES, ph.sum(axis=0))))
+    log(f"slowest window {slow}: nodes={ws[slow].nodes} cells={ws[slow].cells} " +
+        ", ".join(f"{n}={v * 1e3:.1f}ms" for n, v in zip(api.PHASES, ph[slow])))
+
+    ms_per_step = wall * 1000.0 / args.steps
+    windows_per_s = args.windows * args.steps / wall
+    gcups = st.cells * args.steps / wall / 1e9
+    ebar = st.edge_cells / max(st.cells, 1)
+    bytes_per_cell = 4 * (3 + 2 * ebar)            # int32 H, D, I written; H, D read per edge
+    alg_bytes = st.cells * bytes_per_cell          # per launch
+    kms = float(np.mean(kernel_ms))
+    achieved = alg_bytes / (kms / 1000.0) / 1e9
+    line = {
+        "metric": "POA windows/sec on 1 MI355X (bit-exact to minipoa), with GCUPS",
+        "value": round(windows_per_s, 1),
+        "unit": "windows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "gcups": round(gcups, 2),
+        "config": {
+            "workload": "batched uneven windows: 10,000 POA jobs, 5-60 reads each, template "
+                        "lognormal(median 500 bp, sigma 0.5) clipped to 150-2,000 bp, error "
+                        "U(5%,15%) split 1:1:1; minipoa defaults, consensus",
+            "windows": args.windows,
+            "seed": seed,
+            "reads": int(batch.win_off[-1]),
+            "bases": int(batch.seq_off[-1]),
+            "dp_calls": int(st.dp_calls),
+            "band_cells": int(st.cells),
+            "edge_cells_per_cell": round(ebar, 4),
+            "reruns": int(st.reruns),
+            "launches": int(st.launches),
+            "slots": int(st.slots),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "mpoa::k_windows",
+            "kernel_ms": round(kms, 3),
+            "bytes_per_cell": round(bytes_per_cell, 3),
+        },
+    }
+    if not args.no_cpu:
+        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(batch, res.texts, args.cpu_sample, cores)
+    b.free()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

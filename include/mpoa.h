@@ -90,7 +90,11 @@ typedef struct mpoa_batch_stats {
   int64_t cells;         /* sum of 8*block_num over all DP rows */
   int64_t edge_cells;
   int64_t reruns;        /* windows re-run after an arena overflow */
-  double device_ms;      /* device time of the last execute, HIP events */
+  double device_ms;      /* device time of the last execute (all launches and copies) */
+  double kernel_ms;      /* pipeline-kernel time of the last execute, HIP events on the
+                            stream the kernel runs on */
+  int32_t launches;      /* pipeline launches in the last execute (1 + re-run passes) */
+  int32_t slots;         /* resident workgroups of the first launch */
 } mpoa_batch_stats;
 
 /* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window
@@ -100,6 +104,10 @@ typedef struct mpoa_batch_stats {
 int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windows,
                        const int32_t* win_off, const char* seq, const int64_t* seq_off,
                        const double* size_hint, mpoa_batch** out);
+/* Tile shape of the DP rows: `threads` per window workgroup (64 = one wavefront; 64, 128, 256
+ * or 512) and `cols` DP columns per thread per pass (1, 2, 4 or 8). Not every pair is built;
+ * returns MPOA_E_ARG for one that is not. Chosen by the autotuner (poa_amd/autotune.py). */
+int mpoa_batch_set_tile(mpoa_batch* b, int32_t threads, int32_t cols);
 /* Runs the whole per-window pipeline on the device for every window; inputs are already
  * resident. Synchronous. */
 int mpoa_batch_execute(mpoa_batch* b);
@@ -108,6 +116,16 @@ int mpoa_batch_execute(mpoa_batch* b);
 int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_off,
                        mpoa_batch_stats* stats);
 void mpoa_batch_free(mpoa_batch* b);
+
+/* Per-window diagnostics of the last execute: final graph size, DP cells and device time per
+ * pipeline phase (DP plan, DP rows, backtrack, add_path, topological sort, output), in ticks
+ * of the device wall clock (*tick_hz ticks per second). */
+typedef struct mpoa_window_stat {
+  int32_t status, nodes, edges, pad;
+  int64_t cells, edge_cells;
+  int64_t ticks[6];
+} mpoa_window_stat;
+int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);
 
 #ifdef __cplusplus
 }

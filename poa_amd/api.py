@@ -25,8 +25,11 @@ ERRORS = {
 
 EXPORTS = [
     "mpoa_create", "mpoa_destroy", "mpoa_strerror", "mpoa_align_reads", "mpoa_batch_prepare",
-    "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free",
+    "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
+    "mpoa_batch_window_stats",
 ]
+
+PHASES = ("plan", "dp", "backtrack", "add_path", "topsort", "output")
 
 
 class MpoaError(RuntimeError):
@@ -56,10 +59,18 @@ class CallStat(ctypes.Structure):
                 ("edge_cells", ctypes.c_int64)]
 
 
+class WindowStat(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
+                ("ticks", ctypes.c_int64 * 6)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("windows", ctypes.c_int64), ("dp_calls", ctypes.c_int64),
                 ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
-                ("reruns", ctypes.c_int64), ("device_ms", ctypes.c_double)]
+                ("reruns", ctypes.c_int64), ("device_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int32),
+                ("slots", ctypes.c_int32)]
 
 
 _lib = None
@@ -84,6 +95,8 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_results.argtypes = [VP, VP, I64, VP, P(BatchStats)]
         L.mpoa_batch_free.argtypes = [VP]
         L.mpoa_batch_free.restype = None
+        L.mpoa_batch_set_tile.argtypes = [VP, I32, I32]
+        L.mpoa_batch_window_stats.argtypes = [VP, VP, I32, P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -177,10 +190,20 @@ class Batch:
         self.h = h
         self.n = len(self.win_off) - 1
 
+    def set_tile(self, threads: int, cols: int) -> None:
+        _check(lib().mpoa_batch_set_tile(self.h, threads, cols))
+
     def execute(self) -> float:
         """Runs every window on the device; returns device milliseconds."""
         _check(lib().mpoa_batch_execute(self.h))
         return self.stats().device_ms
+
+    def window_stats(self):
+        """(records, seconds per tick) of the last execute."""
+        arr = (WindowStat * max(self.n, 1))()
+        hz = ctypes.c_double()
+        _check(lib().mpoa_batch_window_stats(self.h, arr, self.n, ctypes.byref(hz)))
+        return [arr[i] for i in range(self.n)], 1.0 / hz.value
 
     def stats(self) -> BatchStats:
         st = BatchStats()

@@ -146,7 +146,11 @@ struct mpoa_batch {
   int nt = 64, cpl = 4;
   std::vector<WinResult> res;
   int64_t reruns = 0;
+  bool mem_limited = false;
   double device_ms = 0;
+  double kernel_ms = 0;
+  int launches = 0;
+  int first_slots = 0;
   // per-call statistics (single-window runs)
   DevBuf<CallStat> d_calls;
   int calls_stride = 0;
@@ -185,19 +189,33 @@ size_t slot_bytes_total(const SlotCaps& c) {
 int launch(mpoa_batch* b, const std::vector<int>& wins, const SlotCaps& caps, int max_slots) {
   mpoa_ctx* ctx = b->ctx;
   const size_t per_slot = slot_bytes_total(caps);
-  size_t free_b = 0, total_b = 0;
-  HIPCHK(hipMemGetInfo(&free_b, &total_b));
-  // Keep a quarter of free memory back for the runtime and other users.
-  size_t budget = free_b / 4 * 3;
-  int n_slots = (int)std::min<size_t>({(size_t)max_slots, wins.size(), budget / per_slot});
-  if (n_slots < 1) {
-    g_err = "window too large for device memory";
-    return MPOA_E_OOM;
+  int n_slots;
+  const bool same = b->d_dp.p && caps.nodes == b->caps.nodes && caps.edges == b->caps.edges &&
+                    caps.qmax == b->caps.qmax && caps.path == b->caps.path && caps.dp == b->caps.dp;
+  if (same && (b->n_slots >= std::min<int>(max_slots, (int)wins.size()) || b->mem_limited)) {
+    // Pools of the previous launch fit (or were already as large as memory allows): reuse.
+    n_slots = std::min<int>(b->n_slots, (int)wins.size());
+  } else {
+    b->d_int.release();
+    b->d_byte.release();
+    b->d_dp.release();
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    // Keep a quarter of free memory back for the runtime and other users.
+    const size_t budget = free_b / 4 * 3;
+    n_slots = (int)std::min<size_t>({(size_t)max_slots, wins.size(), budget / per_slot});
+    b->mem_limited = (size_t)n_slots == budget / per_slot && (size_t)n_slots < wins.size();
+    if (n_slots < 1) {
+      g_err = "window too large for device memory";
+      return MPOA_E_OOM;
+    }
+    int rc;
+    if ((rc = b->d_int.alloc((size_t)n_slots * caps.ints_per_slot)) != MPOA_OK) return rc;
+    if ((rc = b->d_byte.alloc((size_t)n_slots * caps.bytes_per_slot)) != MPOA_OK) return rc;
+    if ((rc = b->d_dp.alloc((size_t)n_slots * 3 * caps.dp)) != MPOA_OK) return rc;
+    b->n_slots = n_slots;
+    b->caps = caps;
   }
-  int rc;
-  if ((rc = b->d_int.alloc((size_t)n_slots * caps.ints_per_slot)) != MPOA_OK) return rc;
-  if ((rc = b->d_byte.alloc((size_t)n_slots * caps.bytes_per_slot)) != MPOA_OK) return rc;
-  if ((rc = b->d_dp.alloc((size_t)n_slots * 3 * caps.dp)) != MPOA_OK) return rc;
   HIPCHK(hipMemcpyAsync(b->d_order.p, wins.data(), wins.size() * sizeof(int),
                         hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemsetAsync(b->d_queue.p, 0, sizeof(int), ctx->stream));
@@ -218,14 +236,26 @@ int launch(mpoa_batch* b, const std::vector<int>& wins, const SlotCaps& caps, in
   a.call_stats = b->d_calls.p;
   a.call_stats_stride = b->calls_stride;
   a.call_stats_cap = b->calls_stride;
+  hipEvent_t k0, k1;
+  HIPCHK(hipEventCreate(&k0));
+  HIPCHK(hipEventCreate(&k1));
+  HIPCHK(hipEventRecord(k0, ctx->stream));
   const int lrc = mpoa_launch_windows(a, n_slots, b->nt, b->cpl, ctx->stream);
   if (lrc != 0) {
+    (void)hipEventDestroy(k0);
+    (void)hipEventDestroy(k1);
     g_err = lrc == -2 ? "unsupported tile shape" : "kernel launch failed";
     return lrc == -2 ? MPOA_E_ARG : MPOA_E_DEVICE;
   }
+  HIPCHK(hipEventRecord(k1, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  b->n_slots = n_slots;
-  b->caps = caps;
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, k0, k1));
+  (void)hipEventDestroy(k0);
+  (void)hipEventDestroy(k1);
+  b->kernel_ms += kms;
+  if (b->launches == 0) b->first_slots = n_slots;
+  b->launches++;
   return MPOA_OK;
 }
 
@@ -380,6 +410,8 @@ int mpoa_batch_execute(mpoa_batch* b) {
   const int n = b->n_windows;
   b->res.assign(n, WinResult{});
   b->reruns = 0;
+  b->kernel_ms = 0;
+  b->launches = 0;
   if (n == 0) return MPOA_OK;
   const bool keep_path = b->params.result != 0;
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
@@ -437,6 +469,9 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
     }
     stats->reruns = b->reruns;
     stats->device_ms = b->device_ms;
+    stats->kernel_ms = b->kernel_ms;
+    stats->launches = b->launches;
+    stats->slots = b->first_slots;
   }
   if (!out_off) return MPOA_OK;
   std::vector<uint8_t> raw((size_t)std::max<int64_t>(b->out_total, 1));
@@ -469,6 +504,37 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
 }
 
 void mpoa_batch_free(mpoa_batch* b) { delete b; }
+
+int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz) {
+  if (!b) return MPOA_E_ARG;
+  if (tick_hz) {
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, b->ctx->device) !=
+        hipSuccess)
+      rate_khz = 100000;
+    *tick_hz = rate_khz * 1000.0;
+  }
+  if (!out) return MPOA_OK;
+  for (int w = 0; w < b->n_windows && w < cap && w < (int)b->res.size(); w++) {
+    const WinResult& r = b->res[w];
+    mpoa_window_stat& o = out[w];
+    o.status = r.status;
+    o.nodes = r.nodes;
+    o.edges = r.edges;
+    o.pad = 0;
+    o.cells = r.cells;
+    o.edge_cells = r.edge_cells;
+    for (int k = 0; k < 6; k++) o.ticks[k] = r.ticks[k];
+  }
+  return MPOA_OK;
+}
+
+int mpoa_batch_set_tile(mpoa_batch* b, int32_t threads, int32_t cols) {
+  if (!b || !mpoa_tile_supported(threads, cols)) return MPOA_E_ARG;
+  b->nt = threads;
+  b->cpl = cols;
+  return MPOA_OK;
+}
 
 int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, const char* seq,
                      const int64_t* seq_off, const char* name, const int64_t* name_off,

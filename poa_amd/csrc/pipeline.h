@@ -30,7 +30,7 @@ struct SlotCaps {
 
 // Int32 and byte words a slot's arena takes; must match carve() in pipeline.hip.
 inline __host__ __device__ int64_t slot_ints(int N, int E, int qmax, int path) {
-  return 31LL * N + 8LL * E + 2LL * qmax + 8 + path;
+  return 40LL * N + 8LL * E + 2LL * qmax + 8 + path;
 }
 inline __host__ __device__ int64_t slot_bytes(int N, int E, int qmax) {
   return (int64_t)N + E + 2LL * qmax + 64;
@@ -43,11 +43,15 @@ struct WinDesc {
   int pad;
 };
 
+// Device timers (wall_clock64 ticks) per phase of the window pipeline.
+enum Phase { PH_PLAN = 0, PH_DP, PH_BT, PH_ADD, PH_SORT, PH_OUT, PH_N };
+
 struct WinResult {
   int status;
   int out_len;     // consensus length, or MSA columns
   int64_t cells, edge_cells;
   int nodes, edges;
+  int64_t ticks[PH_N];
 };
 
 struct CallStat {
@@ -79,3 +83,4 @@ struct KernelArgs {
 // thread per pass. Returns 0, or -2 for an unsupported (nt, cpl).
 int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
                         hipStream_t stream);
+bool mpoa_tile_supported(int nt, int cpl);

@@ -46,10 +46,12 @@ __device__ __forceinline__ uint8_t char_of(int code) {
 struct Arena {
   uint8_t* base;
   int *par, *aligned, *ind, *rank, *in_head, *in_tail, *out_head, *out_tail, *in_cnt, *last_read;
+  int* out_cnt;
   int *ie_src, *ie_w, *ie_next, *oe_dst, *oe_w, *oe_next;
   int *rank_id, *t0, *t1, *t2, *hlen, *tlen;
   int *pred_off, *pred_rank, *pred_w, *Bs, *Be, *row_off;
   uint8_t* pred_base;
+  int* rowrec;  // RowRec per rank (8 ints)
   int *Mp, *Sl, *Pl, *Pr, *Ol, *Or, *Ow;
   int *res_from, *res_aligned;
   uint8_t *res_base, *qseq;
@@ -63,9 +65,10 @@ __device__ Arena carve(const KernelArgs& a, int slot) {
   int* p = a.int_pool + (int64_t)slot * c.ints_per_slot;
   auto take = [&](int64_t n) { int* r = p; p += n; return r; };
   const int N = c.nodes, E = c.edges;
+  A.rowrec = take(8LL * N);  // first: 16-byte aligned for the int4 accesses
   A.par = take(N); A.aligned = take((int64_t)NB * N); A.ind = take(N); A.rank = take(N);
   A.in_head = take(N); A.in_tail = take(N); A.out_head = take(N); A.out_tail = take(N);
-  A.in_cnt = take(N); A.last_read = take(N);
+  A.in_cnt = take(N); A.last_read = take(N); A.out_cnt = take(N);
   A.ie_src = take(E); A.ie_w = take(E); A.ie_next = take(E);
   A.oe_dst = take(E); A.oe_w = take(E); A.oe_next = take(E);
   A.rank_id = take(N); A.t0 = take(N); A.t1 = take(N); A.t2 = take(N);
@@ -103,7 +106,11 @@ struct WinShared {
   int job;
   int scan[16];
   int row_bs, row_be;
+  bool q_lds;
+  int64_t ticks[PH_N];
 };
+
+__device__ __forceinline__ int64_t now() { return (int64_t)wall_clock64(); }
 
 // ---------------------------------------------------------------------------------------
 // Graph edits (one lane). Follow graph::add_adj / node_t::add_in_adj / add_out_adj
@@ -120,6 +127,7 @@ __device__ int new_node(Arena& A, WinShared& S, const SlotCaps& c, int base) {
   A.ind[id] = 0;
   A.in_head[id] = A.in_tail[id] = A.out_head[id] = A.out_tail[id] = -1;
   A.in_cnt[id] = 0;
+  A.out_cnt[id] = 0;
   A.last_read[id] = -1;
   return id;
 }
@@ -140,6 +148,7 @@ __device__ void add_adj(Arena& A, WinShared& S, const SlotCaps& c, bool keep_pat
     A.oe_dst[e] = to; A.oe_w[e] = 1; A.oe_next[e] = -1;
     if (A.out_tail[from] == -1) A.out_head[from] = e; else A.oe_next[A.out_tail[from]] = e;
     A.out_tail[from] = e;
+    A.out_cnt[from]++;
   }
   // in-list of `to`
   A.ind[to]++;
@@ -192,12 +201,99 @@ __device__ void add_path(Arena& A, WinShared& S, const SlotCaps& c, bool keep_pa
   add_adj(A, S, c, keep_path, seq_id, anchor, 1, pos, path0);
 }
 
-// graph::topsort op=0 (src/graph.cpp:59-123): LIFO Kahn order seeded with the zero
-// in-degree nodes in ascending id order; then, in rank order, the DP plan (predecessor CSR in
-// node.in order, with rank, weight and base) and hlen (1 + hlen of the max-in-weight
-// predecessor, first wins); then tlen in reverse rank order (1 + tlen of the max-out-weight
-// successor; the sink is 1).
-__device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len) {
+// graph::topsort op=0 (src/graph.cpp:59-123), split into:
+//   1. the LIFO Kahn order (stack seeded with the zero in-degree nodes in ascending id
+//      order), one lane: in LDS when the graph's out-adjacency fits (built there in parallel),
+//      else on the HBM arrays;
+//   2. in parallel over ranks: rank arrays, the DP plan (predecessor CSR in node.in order with
+//      rank, weight and base; per-row records), and the parents of the hlen/tlen trees (the
+//      max-in-weight predecessor / max-out-weight successor, first wins);
+//   3. hlen = 1 + hlen(parent) (source 0) and tlen = 1 + tlen(child) (sink 1) as depths in
+//      those trees, by pointer jumping.
+
+// Block-wide exclusive prefix sum of v (NT threads); *total gets the sum over all threads.
+template <int NT>
+__device__ __forceinline__ int block_excl_sum(int v, WinShared& S, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const int u = __shfl_up(incl, s, 64);
+    if (lane >= s) incl += u;
+  }
+  if (lane == 63) S.scan[wave] = incl;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int w2 = 0; w2 < NT / 64; w2++) {
+    const int t = S.scan[w2];
+    if (w2 < wave) before += t;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// Kahn order into A.rank_id[0..N) using LDS (lds_bytes available at lds). Returns false when
+// the graph does not fit.
+template <int NT>
+__device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
+  const int N = S.n_nodes, E = S.n_out_e;
+  if (N > 32767 || E > 32767 || 2LL * (4LL * N + 2 + E) > lds_bytes) return false;
+  int16_t* deg = reinterpret_cast<int16_t*>(lds);
+  int16_t* stk = deg + N;
+  int16_t* rid = stk + N;
+  int16_t* ooff = rid + N;   // N + 1
+  int16_t* odst = ooff + N + 1;
+  // out-degree prefix sums (node id order)
+  int base = 0;
+  for (int v0 = 0; v0 < N; v0 += NT) {
+    const int v = v0 + threadIdx.x;
+    const int oc = v < N ? A.out_cnt[v] : 0;
+    int tot;
+    const int ex = block_excl_sum<NT>(oc, S, &tot);
+    if (v < N) {
+      ooff[v] = (int16_t)(base + ex);
+      deg[v] = (int16_t)A.in_cnt[v];
+    }
+    base += tot;
+  }
+  if (threadIdx.x == 0) ooff[N] = (int16_t)base;
+  __syncthreads();
+  for (int v = threadIdx.x; v < N; v += NT) {
+    int k = ooff[v];
+    for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) odst[k++] = (int16_t)A.oe_dst[e];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sp = 0;
+    for (int v = 0; v < N; v++)
+      if (deg[v] == 0) stk[sp++] = (int16_t)v;
+    int r = 0;
+    while (sp > 0) {
+      const int u = stk[--sp];
+      rid[r++] = (int16_t)u;
+      const int e1 = ooff[u + 1];
+      for (int e = ooff[u]; e < e1; e++) {
+        const int v = odst[e];
+        if (--deg[v] == 0) stk[sp++] = (int16_t)v;
+      }
+    }
+    S.n_rows = r;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < S.n_rows; r += NT) {
+    const int u = rid[r];
+    A.rank_id[r] = u;
+    A.rank[u] = r;
+  }
+  __syncthreads();
+  return true;
+}
+
+// Kahn order on the HBM arrays (one lane), for graphs whose adjacency does not fit in LDS.
+__device__ void kahn_global(Arena& A, WinShared& S) {
   const int N = S.n_nodes;
   int* deg = A.t0;
   int* stk = A.t1;
@@ -208,42 +304,107 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
   }
   int r = 0;
   while (sp > 0) {
-    int u = stk[--sp];
+    const int u = stk[--sp];
     A.rank[u] = r;
     A.rank_id[r] = u;
     r++;
     for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
-      int v = A.oe_dst[e];
+      const int v = A.oe_dst[e];
       if (--deg[v] == 0) stk[sp++] = v;
     }
   }
-  // plan + hlen
-  int k = 0;
-  A.hlen[0] = 0;
-  for (int rr = 0; rr < r; rr++) {
-    int u = A.rank_id[rr];
-    A.pred_off[rr] = k;
-    int wmax = -1, max_pre = -1;
+  S.n_rows = r;
+}
+
+// Depth of every rank in a parent forest given as par[r] (-1 at roots), by pointer jumping:
+// depth[r] = base + number of links from r to its root. par and depth are rank-indexed HBM
+// arrays; jmp is scratch. Runs on the first wavefront only: within one wavefront memory
+// operations complete in issue order, so a lane never sees a half-updated (depth, jmp) pair.
+template <int NT>
+__device__ void tree_depth(int n, const int* par, int* depth, int* jmp, int base) {
+  if (threadIdx.x < 64) {
+    for (int r = threadIdx.x; r < n; r += 64) {
+      depth[r] = par[r] >= 0 ? 1 : 0;
+      jmp[r] = par[r];
+    }
+    for (;;) {
+      int any = 0;
+      for (int r = threadIdx.x; r < n; r += 64) {
+        const int j = jmp[r];
+        if (j >= 0) {
+          const int jj = jmp[j];
+          if (jj >= 0) {
+            depth[r] += depth[j];  // links r -> j plus j -> jj
+            jmp[r] = jj;
+            any = 1;
+          }
+        }
+      }
+      if (!__any(any)) break;
+    }
+    for (int r = threadIdx.x; r < n; r += 64) depth[r] += base;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
+                         int lds_bytes) {
+  if (!kahn_lds<NT>(A, S, lds, lds_bytes)) {
+    if (threadIdx.x == 0) kahn_global(A, S);
+    __syncthreads();
+  }
+  const int n = S.n_rows;
+  // DP plan in rank order: predecessor CSR offsets (block scan of in-degrees), then each lane
+  // walks its rank's in-list.
+  int base = 0;
+  for (int r0 = 0; r0 < n; r0 += NT) {
+    const int r = r0 + threadIdx.x;
+    const int ic = r < n ? A.in_cnt[A.rank_id[r]] : 0;
+    int tot;
+    const int ex = block_excl_sum<NT>(ic, S, &tot);
+    if (r < n) A.pred_off[r] = base + ex;
+    base += tot;
+  }
+  if (threadIdx.x == 0) A.pred_off[n] = base;
+  __syncthreads();
+  int* hpar = A.t0;  // max-in-weight predecessor rank (hlen tree)
+  int* tchd = A.t1;  // max-out-weight successor rank (tlen tree)
+  for (int r = threadIdx.x; r < n; r += NT) {
+    const int u = A.rank_id[r];
+    const int k0 = A.pred_off[r];
+    int k = k0, wmax = -1, max_pre = -1, pr3[3] = {0, 0, 0}, pb = 0;
     for (int e = A.in_head[u]; e != -1; e = A.ie_next[e]) {
-      int src = A.ie_src[e], w = A.ie_w[e], pr = A.rank[src];
+      const int src = A.ie_src[e], w = A.ie_w[e], pr = A.rank[src];
+      const int b = A.base[src];
       A.pred_rank[k] = pr;
       A.pred_w[k] = w;
-      A.pred_base[k] = A.base[src];
+      A.pred_base[k] = (uint8_t)b;
+      if (k - k0 < 3) {
+        pr3[k - k0] = pr;
+        pb |= b << (8 * (k - k0));
+      }
       k++;
       if (w > wmax) { wmax = w; max_pre = pr; }
     }
-    if (with_len && max_pre != -1) A.hlen[rr] = A.hlen[max_pre] + 1;
-  }
-  A.pred_off[r] = k;
-  if (with_len) {
-    A.tlen[A.rank[1]] = 1;
-    for (int rr = r - 1; rr >= 0; rr--) {
-      int u = A.rank_id[rr];
-      int wmax = -1, max_suc = -1;
+    int4* rec = reinterpret_cast<int4*>(A.rowrec + 8LL * r);
+    int4 hi;
+    hi.x = pr3[0]; hi.y = pr3[1]; hi.z = pr3[2]; hi.w = pb;
+    rec[1] = hi;
+    A.rowrec[8LL * r + 2] = k0;
+    A.rowrec[8LL * r + 3] = k - k0;
+    hpar[r] = max_pre;
+    if (with_len) {
+      int omax = -1, max_suc = -1;
       for (int e = A.out_head[u]; e != -1; e = A.oe_next[e])
-        if (A.oe_w[e] > wmax) { wmax = A.oe_w[e]; max_suc = A.rank[A.oe_dst[e]]; }
-      if (max_suc != -1) A.tlen[rr] = A.tlen[max_suc] + 1;
+        if (A.oe_w[e] > omax) { omax = A.oe_w[e]; max_suc = A.rank[A.oe_dst[e]]; }
+      tchd[r] = max_suc;
     }
+  }
+  __syncthreads();
+  if (with_len) {
+    tree_depth<NT>(n, hpar, A.hlen, A.t2, 0);
+    tree_depth<NT>(n, tchd, A.tlen, A.t2, 1);
   }
 }
 
@@ -331,25 +492,65 @@ __device__ int consensus(Arena& A, WinShared& S, uint8_t* out, int cap) {
 // Banded DP (src/align.cpp:1357-1514).
 // ---------------------------------------------------------------------------------------
 
-// Exclusive prefix max of v over the workgroup's threads (NT threads, NT/64 waves), and the
-// maximum over all threads in *total.
+// Per-row DP record (8 int32, one per rank), written by the topological sort (predecessors)
+// and the band plan (band, storage offset); read once per row, one row ahead.
+//   [0] row_off   [1] Bs | Be << 16   [2] k0 (first pred in the pred CSR)   [3] n_pred
+//   [4..6] ranks of the first three predecessors   [7] their bases, one byte each
+struct RowRec {
+  int row_off, bsbe, k0, np, p0, p1, p2, pb;
+};
+__device__ __forceinline__ RowRec load_rowrec(const int* rr, int r) {
+  const int4* v = reinterpret_cast<const int4*>(rr + 8LL * r);
+  const int4 a = v[0], b = v[1];
+  return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+// LDS ring of the H and D bands of the last RING rows (row r in slot r % RING, column c of the
+// band at offset c - 8*Bs[r]). Rows wider than WR are not cached.
+template <int RING, int WR>
+struct Ring {
+  int H[RING * WR];
+  int D[RING * WR];
+  int row[RING];
+  int bsbe[RING];
+  uint8_t q[4 * WR + 64];  // query codes (sentinel first) when the read fits
+};
+
+// Orders this workgroup's LDS accesses across a row boundary (no wait on global stores).
+template <int NT>
+__device__ __forceinline__ void lds_barrier() {
+  if (NT == 64) {
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+}
+
+// Inclusive prefix max across a wavefront with DPP row shifts and row broadcasts.
+__device__ __forceinline__ int wave_incl_max(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(SCAN_ID, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = max(v, __builtin_amdgcn_update_dpp(SCAN_ID, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = max(v, __builtin_amdgcn_update_dpp(SCAN_ID, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = max(v, __builtin_amdgcn_update_dpp(SCAN_ID, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = max(v, __builtin_amdgcn_update_dpp(SCAN_ID, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, __builtin_amdgcn_update_dpp(SCAN_ID, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return v;
+}
+
+// Exclusive prefix max of v over the workgroup's threads, and the maximum over all of them.
 template <int NT>
 __device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int incl = v;
-#pragma unroll
-  for (int s = 1; s < 64; s <<= 1) {
-    const int u = __shfl_up(incl, s, 64);
-    if (lane >= s) incl = max(incl, u);
-  }
-  int excl = __shfl_up(incl, 1, 64);
+  const int incl = wave_incl_max(v);
+  int excl = __builtin_amdgcn_update_dpp(SCAN_ID, incl, 0x138, 0xf, 0xf, false);  // wave_shr:1
   if (lane == 0) excl = SCAN_ID;
+  const int wtot = __builtin_amdgcn_readlane(incl, 63);
   if (NT == 64) {
-    *total = __shfl(incl, 63, 64);
+    *total = wtot;
     return excl;
   }
-  if (lane == 63) S.scan[wave] = incl;
-  __syncthreads();
+  if (lane == 0) S.scan[wave] = wtot;
+  lds_barrier<NT>();
   int before = SCAN_ID, all = SCAN_ID;
 #pragma unroll
   for (int w2 = 0; w2 < NT / 64; w2++) {
@@ -357,52 +558,119 @@ __device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total) {
     if (w2 < wave) before = max(before, t);
     all = max(all, t);
   }
-  __syncthreads();
+  lds_barrier<NT>();
   *total = all;
   return max(excl, before);
 }
 
+// A predecessor row's band and whether its H and D are in the LDS ring (slot p % RING).
+struct PredBand {
+  int bs, be;
+  bool lds;
+};
+
+template <int RING, int WR>
+__device__ __forceinline__ PredBand pred_band(const Arena& A, Ring<RING, WR>& R, int i, int p) {
+  PredBand pb;
+  const int slot = p % RING;
+  if (i - p < RING && R.row[slot] == p) {
+    const int bsbe = R.bsbe[slot];
+    pb.bs = bsbe & 0xffff;
+    pb.be = bsbe >> 16;
+    pb.lds = true;
+  } else {
+    pb.bs = A.Bs[p];
+    pb.be = A.Be[p];
+    pb.lds = false;
+  }
+  return pb;
+}
+
+// M/D-source accumulation of one predecessor row into a thread's CPL columns. Hrow/Drow point
+// at the first stored column (8*bs) of the predecessor's band; every index used is
+// non-negative and inside the band.
+template <int CPL, class Ptr>
+__device__ __forceinline__ void accumulate(Ptr Hrow, Ptr Drow, int pofs, int cb, int c0,
+                                           int mlo, int mhi, int dlo, int dhi, int pbase,
+                                           const int* qc, int match, int mismatch, int o, int e,
+                                           int* h, int* d) {
+#pragma unroll
+  for (int l = 0; l < CPL; l++) {
+    const int c = cb + l;
+    if (c >= mlo && c < mhi) {
+      const int hd = Hrow[max(c - 1 - pofs, 0)];
+      const int diag = c == c0 ? NEG_INF : hd;
+      h[l] = max(h[l], diag + (pbase == qc[l] ? match : mismatch));
+    }
+    if (c >= dlo && c < dhi) {
+      const int j = max(c - pofs, 0);
+      d[l] = max(d[l], max(Drow[j] + e, Hrow[j] + o));
+    }
+  }
+}
+
 // One DP row i (local) for rows in rank window [beg_i, end_i]: M/D sources from every
-// in-window predecessor under the per-block band conditions, then the I/H scan computed as
-// an exact prefix max: I[j] = j*e + max(NEG_INF, max_{k<j} (G[k] + o - (k+1)*e)),
-// G = max(M-source, D-source), H = max(G, I). This equals the serial recurrence
-// I[j] = max(I[j-1]+e, H[j-1]+o) because o <= e (gap_open <= 0) and int32 never wraps here.
-template <int NT, int CPL>
-__device__ void dp_row(const Arena& A, WinShared& S, const DpScores& sc, int i) {
+// in-window predecessor under the per-block band conditions of src/align.cpp:1467 and :1484
+// (contiguous column ranges), then the I/H scan as an exact prefix max:
+//   I[j] = j*e + max(NEG_INF, max_{k<j} (G[k] + o - (k+1)*e)),  H = max(G, I),
+// G = max(M-source, D-source). This equals the serial recurrence of align.cpp:1507-1514,
+// I[j] = max(I[j-1]+e, H[j-1]+o), because o <= e (gap_open <= 0) and int32 never wraps here.
+// Writes H, D, I (and the NEG_INF sentinel block) to HBM and H, D to the LDS ring.
+// Returns the first argmax column (local) of the row's H, for the adaptive band.
+template <int NT, int CPL, int RING, int WR>
+__device__ int dp_row(const Arena& A, WinShared& S, Ring<RING, WR>& R, const DpScores& sc,
+                      int i, const RowRec& rr, int bs, int be, bool want_argmax) {
   const int tid = threadIdx.x;
-  const int Bsi = A.Bs[i], Bei = A.Be[i];
-  const int c0 = Bsi * Q, W = (Bei - Bsi - 1) * Q, c1 = c0 + W;
-  const int rowi = A.row_off[i];
-  int* Hi = A.H + rowi - c0;  // indexed by absolute column
-  int* Di = A.D + rowi - c0;
-  int* Ii = A.I + rowi - c0;
-  const int k0 = A.pred_off[S.beg_i + i], k1 = A.pred_off[S.beg_i + i + 1];
+  const int c0 = bs * Q, W = (be - bs - 1) * Q, c1 = c0 + W;
+  int* Hi = A.H + rr.row_off;  // column c at Hi[c - c0]
+  int* Di = A.D + rr.row_off;
+  int* Ii = A.I + rr.row_off;
+  const int slot_i = i % RING;
+  const bool cache_i = W <= WR;
+  int* RHi = R.H + slot_i * WR;  // column c at RHi[c - c0]
+  int* RDi = R.D + slot_i * WR;
   const int o = sc.o1, e = sc.e1;
+  const int beg = S.beg_i;
   int carry = SCAN_ID;
+  int best = SCAN_ID, best_j = 0x7fffffff;
   for (int pb = c0; pb < c1; pb += NT * CPL) {
     const int cb = pb + tid * CPL;
     int h[CPL], d[CPL];
 #pragma unroll
     for (int l = 0; l < CPL; l++) h[l] = d[l] = NEG_INF;
-    for (int k = k0; k < k1; k++) {
-      const int p = A.pred_rank[k] - S.beg_i;
-      if (p < 0) continue;
-      const int pbase = p == 0 ? CODE_N : A.pred_base[k];
-      const int Bsp = A.Bs[p], Bep = A.Be[p];
-      const int pofs = Bsp * Q;
-      const int* Hp = A.H + A.row_off[p] - pofs;
-      const int* Dp = A.D + A.row_off[p] - pofs;
-      const int mlo = max(c0, Bsp == 0 ? 0 : (Bsp + 1) * Q), mhi = min(c1, (Bep - 1) * Q);
-      const int dlo = max(c0, Bsp * Q), dhi = min(c1, (Bep - 1) * Q);
+    int qc[CPL];
+    const bool q_lds = S.q_lds;
 #pragma unroll
-      for (int l = 0; l < CPL; l++) {
-        const int c = cb + l;
-        if (c >= mlo && c < mhi) {
-          const int diag = c == c0 ? NEG_INF : Hp[c - 1];
-          const int qc = c == 0 ? CODE_N : A.qseq[c - 1];
-          h[l] = max(h[l], diag + (pbase == qc ? sc.match : sc.mismatch));
-        }
-        if (c >= dlo && c < dhi) d[l] = max(d[l], max(Dp[c] + e, Hp[c] + o));
+    for (int l = 0; l < CPL; l++) {
+      const int c = cb + l;
+      const int ci = min(max(c - 1, 0), c1 - 1);
+      const int v = q_lds ? R.q[ci] : A.qseq[ci];
+      qc[l] = (c > 0 && c < c1) ? v : CODE_N;
+    }
+    for (int k = 0; k < rr.np; k++) {
+      int prank, pbase;
+      if (k < 3) {
+        prank = k == 0 ? rr.p0 : k == 1 ? rr.p1 : rr.p2;
+        pbase = (rr.pb >> (8 * k)) & 0xff;
+      } else {
+        prank = A.pred_rank[rr.k0 + k];
+        pbase = A.pred_base[rr.k0 + k];
+      }
+      const int p = prank - beg;
+      if (p < 0) continue;
+      if (p == 0) pbase = CODE_N;
+      const PredBand pb = pred_band<RING, WR>(A, R, i, p);
+      const int mlo = max(c0, pb.bs == 0 ? 0 : (pb.bs + 1) * Q), mhi = min(c1, (pb.be - 1) * Q);
+      const int dlo = max(c0, pb.bs * Q), dhi = min(c1, (pb.be - 1) * Q);
+      const int pofs = pb.bs * Q;
+      if (pb.lds) {
+        const int slot = p % RING;
+        accumulate<CPL>(R.H + slot * WR, R.D + slot * WR, pofs, cb, c0, mlo, mhi, dlo, dhi,
+                        pbase, qc, sc.match, sc.mismatch, o, e, h, d);
+      } else {
+        const int off = A.row_off[p];
+        accumulate<CPL>(A.H + off, A.D + off, pofs, cb, c0, mlo, mhi, dlo, dhi, pbase, qc,
+                        sc.match, sc.mismatch, o, e, h, d);
       }
     }
     int g[CPL], t[CPL];
@@ -416,22 +684,59 @@ __device__ void dp_row(const Arena& A, WinShared& S, const DpScores& sc, int i) 
     }
     int total;
     int run = max(block_excl_max<NT>(lane_max, S, &total), carry);
+    if (cb < c1) {  // W and cb - c0 are multiples of CPL: a thread's columns are all in or out
+      int hv[CPL], iv[CPL];
 #pragma unroll
-    for (int l = 0; l < CPL; l++) {
-      const int c = cb + l;
-      if (c < c1) {
-        const int j = c - c0;
-        const int iv = j * e + max(NEG_INF, run);
-        Hi[c] = max(g[l], iv);
-        Di[c] = d[l];
-        Ii[c] = iv;
+      for (int l = 0; l < CPL; l++) {
+        const int j = cb + l - c0;
+        iv[l] = j * e + max(NEG_INF, run);
+        hv[l] = max(g[l], iv[l]);
+        run = max(run, t[l]);
+        if (want_argmax && hv[l] > best) { best = hv[l]; best_j = j; }
       }
-      run = max(run, t[l]);
+      const int j0 = cb - c0;
+#pragma unroll
+      for (int l = 0; l < CPL; l++) {
+        Hi[j0 + l] = hv[l];
+        Di[j0 + l] = d[l];
+        Ii[j0 + l] = iv[l];
+      }
+      if (cache_i) {
+#pragma unroll
+        for (int l = 0; l < CPL; l++) {
+          RHi[j0 + l] = hv[l];
+          RDi[j0 + l] = d[l];
+        }
+      }
     }
     carry = max(carry, total);
   }
   // NEG_INF sentinel block after the band (align.cpp:1505-1506,1515).
-  if (tid < Q) Hi[c1 + tid] = Di[c1 + tid] = Ii[c1 + tid] = NEG_INF;
+  if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = NEG_INF;
+  if (tid == 0) {
+    R.row[slot_i] = cache_i ? i : -1;
+    R.bsbe[slot_i] = bs | (be << 16);
+  }
+  if (!want_argmax) return 0;
+  // First argmax over the row (value desc, column asc).
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    const int ov = __shfl_xor(best, s, 64), oj = __shfl_xor(best_j, s, 64);
+    if (ov > best || (ov == best && oj < best_j)) { best = ov; best_j = oj; }
+  }
+  if (NT > 64) {
+    const int wave = tid >> 6;
+    lds_barrier<NT>();
+    if ((tid & 63) == 0) { S.scan[2 * wave] = best; S.scan[2 * wave + 1] = best_j; }
+    lds_barrier<NT>();
+    best = S.scan[0]; best_j = S.scan[1];
+    for (int w2 = 1; w2 < NT / 64; w2++) {
+      const int ov = S.scan[2 * w2], oj = S.scan[2 * w2 + 1];
+      if (ov > best || (ov == best && oj < best_j)) { best = ov; best_j = oj; }
+    }
+    lds_barrier<NT>();
+  }
+  return best_j == 0x7fffffff ? 0 : best_j;
 }
 
 // Static band per row (src/align.cpp:1378-1394) and ragged row offsets; all lanes.
@@ -478,7 +783,13 @@ __device__ bool plan_static(const Arena& A, WinShared& S, const DpScores& sc, co
       all += S.scan[w2];
     }
     __syncthreads();
-    if (r < n) A.row_off[r] = (int)(base + before + v - len);
+    if (r < n) {
+      const int off = (int)(base + before + v - len);
+      A.row_off[r] = off;
+      int* rec = A.rowrec + 8LL * (beg + r);
+      rec[0] = off;
+      rec[1] = A.Bs[r] | (A.Be[r] << 16);
+    }
     base += all;
   }
   if (threadIdx.x == 0) {
@@ -677,122 +988,137 @@ __device__ void adaptive_update(const Arena& A, WinShared& S, int i, int max_j, 
   }
 }
 
-// First argmax of H over the row's computed columns (strict >, from column 0).
+// LDS ring shape per workgroup size: RING rows of WR columns (int32 H and D).
 template <int NT>
-__device__ int row_argmax(const Arena& A, WinShared& S, int i) {
-  const int W = (A.Be[i] - A.Bs[i] - 1) * Q;
-  const int* Hi = A.H + A.row_off[i];
-  int best = SCAN_ID, bj = 0x7fffffff;
-  for (int j = threadIdx.x; j < W; j += NT) {
-    const int v = Hi[j];
-    if (v > best) { best = v; bj = j; }
-  }
-  // reduce (value desc, index asc)
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int s = 32; s > 0; s >>= 1) {
-    const int ov = __shfl_xor(best, s, 64), oj = __shfl_xor(bj, s, 64);
-    if (ov > best || (ov == best && oj < bj)) { best = ov; bj = oj; }
-  }
-  if (NT > 64) {
-    __syncthreads();
-    if (lane == 0) { S.scan[2 * wave] = best; S.scan[2 * wave + 1] = bj; }
-    __syncthreads();
-    best = S.scan[0]; bj = S.scan[1];
-    for (int w2 = 1; w2 < NT / 64; w2++) {
-      const int ov = S.scan[2 * w2], oj = S.scan[2 * w2 + 1];
-      if (ov > best || (ov == best && oj < bj)) { best = ov; bj = oj; }
-    }
-    __syncthreads();
-  }
-  return bj == 0x7fffffff ? 0 : bj;
-}
+struct RingShape {
+  static constexpr int RING = 8;
+  static constexpr int WR = NT == 64 ? 384 : NT == 128 ? 512 : NT == 256 ? 1280 : 2048;
+};
 
 // One poa() call on the device graph: rows [beg_i, end_i] of the current rank order.
-template <int NT, int CPL>
-__device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, bool adaptive) {
+template <int NT, int CPL, int RING, int WR>
+__device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING, WR>& R,
+                         bool adaptive) {
   const DpScores& sc = a.scores;
-  const int n = S.n_rows, m = S.m;
+  const int n = S.n_rows, m = S.m, beg = S.beg_i;
+  const int tid = threadIdx.x;
   int w = sc.b;
   if (sc.f) w += m / sc.f;
+  const int64_t t0 = now();
+  const int full_len = (m / Q + 2) * Q;
   if (!adaptive) {
     if (!plan_static<NT>(A, S, sc, a.caps, w)) {
-      if (threadIdx.x == 0) S.status = MPOA_E_OVERFLOW;
+      if (tid == 0) S.status = MPOA_E_OVERFLOW;
       __syncthreads();
       return;
     }
   } else {
     // Full-width rows (align.cpp:1385-1390); adaptive state reset (align.cpp:1377).
-    const int len = (m / Q + 2) * Q;
-    if ((int64_t)len * n > a.caps.dp) {
-      if (threadIdx.x == 0) S.status = MPOA_E_OVERFLOW;
+    if ((int64_t)full_len * n > a.caps.dp) {
+      if (tid == 0) S.status = MPOA_E_OVERFLOW;
       __syncthreads();
       return;
     }
-    for (int r = threadIdx.x; r < n; r += NT) {
-      A.row_off[r] = r * len;
+    for (int r = tid; r < n; r += NT) {
+      A.row_off[r] = r * full_len;
       A.Mp[r] = 0; A.Sl[r] = 0; A.Pl[r] = m; A.Pr[r] = 0; A.Ol[r] = 0; A.Or[r] = 0; A.Ow[r] = 0;
     }
-    if (threadIdx.x == 0) {
-      A.row_off[n] = n * len;
+    if (tid == 0) {
+      A.row_off[n] = n * full_len;
       adaptive_band_row(A, S, 0, w);
     }
     __syncthreads();
   }
   // Row 0 (align.cpp:1415-1425): M, D = NEG_INF over blocks [Bs0, Be0) from the row start;
-  // M[0][0] = 0. I of row 0 is never read; it is set to NEG_INF.
+  // M[0][0] = 0. I of row 0 is never read; it is set to NEG_INF. Bs0 is always 0.
   {
-    const int len0 = A.Be[0] * Q;
+    const int be0 = A.Be[0];
+    const int len0 = be0 * Q, w0 = (be0 - 1) * Q;
     int* H0 = A.H + A.row_off[0];
     int* D0 = A.D + A.row_off[0];
     int* I0 = A.I + A.row_off[0];
-    for (int j = threadIdx.x; j < len0; j += NT) {
+    const bool cache0 = w0 <= WR;
+    for (int j = tid; j < len0; j += NT) {
       H0[j] = j == 0 ? 0 : NEG_INF;
       D0[j] = NEG_INF;
       I0[j] = NEG_INF;
+      if (cache0 && j < w0) {
+        R.H[j] = j == 0 ? 0 : NEG_INF;
+        R.D[j] = NEG_INF;
+      }
+    }
+    if (tid < RING) {
+      R.row[tid] = tid == 0 && cache0 ? 0 : -1;
+      R.bsbe[tid] = be0 << 16;
     }
   }
   __syncthreads();
+  const int64_t t1 = now();
+  if (tid == 0) S.ticks[PH_PLAN] += t1 - t0;
   int64_t cells = 0, ecells = 0;
+  RowRec next = n > 1 ? load_rowrec(A.rowrec, beg + 1) : RowRec{};
   for (int i = 1; i < n; i++) {
+    RowRec rr = next;
+    if (i + 1 < n) next = load_rowrec(A.rowrec, beg + i + 1);
+    int bs, be;
     if (adaptive) {
-      if (threadIdx.x == 0) adaptive_band_row(A, S, i, w);
-      __syncthreads();
+      if (tid == 0) {
+        adaptive_band_row(A, S, i, w);
+        S.row_bs = A.Bs[i];
+        S.row_be = A.Be[i];
+      }
+      lds_barrier<NT>();
+      bs = S.row_bs;
+      be = S.row_be;
+      rr.row_off = i * full_len;
+    } else {
+      bs = rr.bsbe & 0xffff;
+      be = rr.bsbe >> 16;
     }
-    if (A.Be[i] - A.Bs[i] - 1 < 1) {
-      if (threadIdx.x == 0) S.status = MPOA_E_BAND;
+    const int W = (be - bs - 1) * Q;
+    if (W < Q) {
+      if (tid == 0) S.status = MPOA_E_BAND;
       __syncthreads();
       return;
     }
-    dp_row<NT, CPL>(A, S, sc, i);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int Wd = (A.Be[i] - A.Bs[i] - 1) * Q;
-      int np = 0;
-      for (int k = A.pred_off[S.beg_i + i]; k < A.pred_off[S.beg_i + i + 1]; k++)
-        np += A.pred_rank[k] - S.beg_i >= 0;
-      cells += Wd;
-      ecells += (int64_t)np * Wd;
+    // Predecessors outside the ring are read from HBM: with several waves, their stores must
+    // have completed first.
+    int np_in = 0;
+    bool far = false;
+    for (int k = 0; k < rr.np; k++) {
+      const int prank = k == 0 ? rr.p0 : k == 1 ? rr.p1 : k == 2 ? rr.p2 : A.pred_rank[rr.k0 + k];
+      const int p = prank - beg;
+      if (p < 0) continue;
+      np_in++;
+      far |= !(i - p < RING && R.row[p % RING] == p);
     }
+    if (NT > 64 && far) __syncthreads();
+    const int mj = dp_row<NT, CPL, RING, WR>(A, S, R, sc, i, rr, bs, be, adaptive);
+    lds_barrier<NT>();
+    cells += W;
+    ecells += (int64_t)np_in * W;
     if (adaptive) {
-      const int mj = row_argmax<NT>(A, S, i);
-      if (threadIdx.x == 0) adaptive_update(A, S, i, mj, w);
-      __syncthreads();
+      if (tid == 0) adaptive_update(A, S, i, mj, w);
+      lds_barrier<NT>();
     }
   }
-  if (threadIdx.x == 0) {
+  __syncthreads();
+  if (tid == 0) {
+    const int64_t t2 = now();
+    S.ticks[PH_DP] += t2 - t1;
     S.cells += cells;
     S.edge_cells += ecells;
     S.score = A.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)];
     const int rc = backtrack(A, S, sc);
     if (rc != MPOA_OK) S.status = rc;
+    S.ticks[PH_BT] += now() - t2;
   }
   __syncthreads();
 }
 
 // The whole per-window pipeline.
-template <int NT, int CPL>
-__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
+template <int NT, int CPL, int RING, int WR>
+__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING, WR>& R, int w) {
   const WinDesc& wd = a.win[w];
   const int tid = threadIdx.x;
   const bool keep_path = a.scores.result != 0;
@@ -801,6 +1127,7 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
     S.status = MPOA_OK;
     S.n_nodes = 0; S.n_in_e = 0; S.n_out_e = 0;
     S.cells = 0; S.edge_cells = 0;
+    for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
     new_node(A, S, a.caps, CODE_N);  // source 0 (graph.cpp:129-135)
     new_node(A, S, a.caps, CODE_N);  // sink 1
   }
@@ -817,9 +1144,14 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
     }
     // Query: sentinel N, codes, N padding to col_size (align.cpp:1362-1373).
     const int col_size = (m + 1 + Q - 1) / Q * Q;
-    for (int j = tid; j < col_size + Q; j += NT)
-      A.qseq[j] = j >= 1 && j <= qlen ? (uint8_t)code_of(a.read_seq[q0 + j - 1]) : (uint8_t)CODE_N;
-    if (tid == 0) { S.read = r; S.qlen = qlen; S.m = m; }
+    const bool q_lds = col_size + Q <= (int)sizeof(R.q);
+    for (int j = tid; j < col_size + Q; j += NT) {
+      const uint8_t cq =
+          j >= 1 && j <= qlen ? (uint8_t)code_of(a.read_seq[q0 + j - 1]) : (uint8_t)CODE_N;
+      A.qseq[j] = cq;
+      if (q_lds) R.q[j] = cq;
+    }
+    if (tid == 0) { S.read = r; S.qlen = qlen; S.m = m; S.q_lds = q_lds; }
     __syncthreads();
     if (S.n_nodes <= 2) {
       // First read: source then insertions (align.cpp:1747-1753).
@@ -835,7 +1167,7 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
         S.n_rows = S.end_i - S.beg_i + 1;
       }
       __syncthreads();
-      poa_call<NT, CPL>(a, A, S, a.scores.ab_band != 0 && a.scores.f > 0);
+      poa_call<NT, CPL, RING, WR>(a, A, S, R, a.scores.ab_band != 0 && a.scores.f > 0);
       if (tid == 0 && a.call_stats && r - 1 < a.call_stats_cap) {
         CallStat& cs = a.call_stats[(int64_t)w * a.call_stats_stride + (r - 1)];
         cs.score = S.score;
@@ -846,17 +1178,27 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
     __syncthreads();
     if (S.status != MPOA_OK) break;
     if (tid == 0) {
+      const int64_t ta = now();
       add_path(A, S, a.caps, keep_path, r, m, path0);
-      if (S.status == MPOA_OK) {
-        if (r + 1 == wd.n_reads) S.msa_cols = topsort1(A, S);
-        else topsort0(A, S, a.caps, with_len);
+      S.ticks[PH_ADD] += now() - ta;
+    }
+    __syncthreads();
+    if (S.status == MPOA_OK) {
+      const int64_t tb = now();
+      if (r + 1 == wd.n_reads) {
+        if (tid == 0) S.msa_cols = topsort1(A, S);
+      } else {
+        topsort0<NT>(A, S, a.caps, with_len, reinterpret_cast<uint8_t*>(R.H),
+                     (int)(sizeof(R.H) + sizeof(R.D)));
       }
+      if (tid == 0) S.ticks[PH_SORT] += now() - tb;
     }
     path0 += qlen;
     __syncthreads();
     if (S.status != MPOA_OK) break;
   }
   // Outputs.
+  const int64_t t_out = now();
   if (S.status == MPOA_OK) {
     if (a.scores.result == 0) {
       if (tid == 0) {
@@ -894,7 +1236,9 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
     }
   }
   if (tid == 0) {
+    S.ticks[PH_OUT] += now() - t_out;
     WinResult& res = a.result[w];
+    for (int k = 0; k < PH_N; k++) res.ticks[k] = S.ticks[k];
     res.status = S.status;
     res.out_len = S.status == MPOA_OK ? S.cons_len : 0;
     res.cells = S.cells;
@@ -908,6 +1252,8 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, int w) {
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
   __shared__ WinShared S;
+  constexpr int RING = RingShape<NT>::RING, WR = RingShape<NT>::WR;
+  __shared__ Ring<RING, WR> R;
   Arena A = carve(a, blockIdx.x);
   for (;;) {
     if (threadIdx.x == 0) S.job = atomicAdd(a.queue, 1);
@@ -915,11 +1261,21 @@ __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
     const int job = S.job;
     __syncthreads();
     if (job >= a.n_jobs) break;
-    run_window<NT, CPL>(a, A, S, a.order[job]);
+    run_window<NT, CPL, RING, WR>(a, A, S, R, a.order[job]);
   }
 }
 
 }  // namespace mpoa
+
+#define MPOA_TILES(X) \
+  X(64, 1) X(64, 2) X(64, 4) X(64, 8) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4) X(512, 2)
+
+bool mpoa_tile_supported(int nt, int cpl) {
+#define MPOA_T(NT_, CPL_) if (nt == NT_ && cpl == CPL_) return true;
+  MPOA_TILES(MPOA_T)
+#undef MPOA_T
+  return false;
+}
 
 // Host-side launch (pipeline.h).
 int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
@@ -930,10 +1286,7 @@ int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
     hipLaunchKernelGGL((k_windows<NT_, CPL_>), dim3(n_slots), dim3(NT_), 0, stream, a);     \
     return hipGetLastError() == hipSuccess ? 0 : -1;                                         \
   }
-  MPOA_L(64, 1) MPOA_L(64, 2) MPOA_L(64, 4) MPOA_L(64, 8)
-  MPOA_L(128, 2) MPOA_L(128, 4)
-  MPOA_L(256, 1) MPOA_L(256, 2) MPOA_L(256, 4)
-  MPOA_L(512, 2)
+  MPOA_TILES(MPOA_L)
 #undef MPOA_L
   return -2;
 }
