@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,7 +22,8 @@ using mpoa::WinResult;
 
 struct mpoa_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // small-window launches, copies
+  hipStream_t stream2 = nullptr;  // large-window launches, concurrent with the first
   int n_cu = 0;
   size_t mem_total = 0;
 };
@@ -84,9 +86,10 @@ struct WinInfo {
   int64_t qtot;
 };
 
-// Capacities estimated from the reads alone. Nodes: the longest read plus a quarter of the
-// other bases (final graphs of 5-15% error reads hold ~0.1 node per input base); DP cells:
-// nodes times the static band width 2w + 64. The device checks every capacity exactly.
+// Capacities estimated from the reads alone. Nodes: the longest read plus 0.16 node per other
+// base (reads at up to 15% error add at most ~0.12 new nodes per base); DP cells: nodes times
+// the static band width 2w + 64. The device checks every capacity exactly and a window that
+// outgrows one is re-run with worst-case capacities.
 struct Estimate {
   int nodes, edges;
   int64_t dp;
@@ -94,7 +97,7 @@ struct Estimate {
 
 Estimate estimate(const WinInfo& w, const DpScores& s, bool full_width) {
   Estimate e;
-  int64_t n = 2 + w.qmax + (int64_t)std::ceil(0.25 * (double)(w.qtot - w.qmax));
+  int64_t n = 2 + w.qmax + (int64_t)std::ceil(0.16 * (double)(w.qtot - w.qmax));
   n = std::min<int64_t>(n, 2 + w.qtot);
   e.nodes = (int)n;
   e.edges = (int)std::min<int64_t>(3 * n + w.n_reads, w.qtot + w.n_reads + 2);
@@ -121,6 +124,21 @@ Estimate worst_case(const WinInfo& w) {
 
 }  // namespace
 
+// One persistent launch configuration: a set of windows, a tile shape and slot capacities.
+struct Bin {
+  int nt = 64, cpl = 4;
+  std::vector<int> wins;  // processing order
+  SlotCaps caps{};
+  int n_slots = 0;        // allocated slots
+  bool mem_limited = false;
+  DevBuf<int> d_int;
+  DevBuf<uint8_t> d_byte;
+  DevBuf<int> d_dp;
+  DevBuf<int> d_order;
+  DevBuf<int> d_queue;
+  int used_slots = 0;     // slots of the last launch
+};
+
 struct mpoa_batch {
   mpoa_ctx* ctx = nullptr;
   mpoa_params params{};
@@ -135,18 +153,11 @@ struct mpoa_batch {
   DevBuf<int64_t> d_off;
   DevBuf<WinDesc> d_win;
   DevBuf<WinResult> d_res;
-  DevBuf<int> d_order;
-  DevBuf<int> d_queue;
   DevBuf<uint8_t> d_out;
-  DevBuf<int> d_int;
-  DevBuf<uint8_t> d_byte;
-  DevBuf<int> d_dp;
-  SlotCaps caps{};
-  int n_slots = 0;
-  int nt = 64, cpl = 4;
+  Bin small, large, rerun;
+  bool split = true;          // large windows in their own concurrent launch
   std::vector<WinResult> res;
   int64_t reruns = 0;
-  bool mem_limited = false;
   double device_ms = 0;
   double kernel_ms = 0;
   int launches = 0;
@@ -185,76 +196,72 @@ size_t slot_bytes_total(const SlotCaps& c) {
   return (size_t)c.ints_per_slot * 4 + (size_t)c.bytes_per_slot + (size_t)c.dp * 12;
 }
 
-// One launch of the device pipeline over `wins` with capacities `caps`.
-int launch(mpoa_batch* b, const std::vector<int>& wins, const SlotCaps& caps, int max_slots) {
-  mpoa_ctx* ctx = b->ctx;
+// Allocates (or reuses) a bin's slot pools for its windows and capacities, within budget
+// bytes of device memory. Returns MPOA_OK or an error.
+int provision(Bin& bin, const SlotCaps& caps, int max_slots, size_t budget) {
   const size_t per_slot = slot_bytes_total(caps);
-  int n_slots;
-  const bool same = b->d_dp.p && caps.nodes == b->caps.nodes && caps.edges == b->caps.edges &&
-                    caps.qmax == b->caps.qmax && caps.path == b->caps.path && caps.dp == b->caps.dp;
-  if (same && (b->n_slots >= std::min<int>(max_slots, (int)wins.size()) || b->mem_limited)) {
-    // Pools of the previous launch fit (or were already as large as memory allows): reuse.
-    n_slots = std::min<int>(b->n_slots, (int)wins.size());
-  } else {
-    b->d_int.release();
-    b->d_byte.release();
-    b->d_dp.release();
-    size_t free_b = 0, total_b = 0;
-    HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    // Keep a quarter of free memory back for the runtime and other users.
-    const size_t budget = free_b / 4 * 3;
-    n_slots = (int)std::min<size_t>({(size_t)max_slots, wins.size(), budget / per_slot});
-    b->mem_limited = (size_t)n_slots == budget / per_slot && (size_t)n_slots < wins.size();
-    if (n_slots < 1) {
-      g_err = "window too large for device memory";
-      return MPOA_E_OOM;
-    }
-    int rc;
-    if ((rc = b->d_int.alloc((size_t)n_slots * caps.ints_per_slot)) != MPOA_OK) return rc;
-    if ((rc = b->d_byte.alloc((size_t)n_slots * caps.bytes_per_slot)) != MPOA_OK) return rc;
-    if ((rc = b->d_dp.alloc((size_t)n_slots * 3 * caps.dp)) != MPOA_OK) return rc;
-    b->n_slots = n_slots;
-    b->caps = caps;
+  const int want = std::min<int>(max_slots, (int)bin.wins.size());
+  const bool same = bin.d_dp.p && caps.nodes == bin.caps.nodes && caps.edges == bin.caps.edges &&
+                    caps.qmax == bin.caps.qmax && caps.path == bin.caps.path && caps.dp == bin.caps.dp;
+  if (same && (bin.n_slots >= want || bin.mem_limited)) {
+    bin.used_slots = std::min(bin.n_slots, want);
+    return MPOA_OK;
   }
-  HIPCHK(hipMemcpyAsync(b->d_order.p, wins.data(), wins.size() * sizeof(int),
-                        hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemsetAsync(b->d_queue.p, 0, sizeof(int), ctx->stream));
+  bin.d_int.release();
+  bin.d_byte.release();
+  bin.d_dp.release();
+  const size_t fit = budget / per_slot;
+  const int n_slots = (int)std::min<size_t>((size_t)want, fit);
+  if (n_slots < 1) {
+    g_err = "window too large for device memory";
+    return MPOA_E_OOM;
+  }
+  bin.mem_limited = (size_t)n_slots == fit && n_slots < want;
+  int rc;
+  if ((rc = bin.d_int.alloc((size_t)n_slots * caps.ints_per_slot)) != MPOA_OK) return rc;
+  if ((rc = bin.d_byte.alloc((size_t)n_slots * caps.bytes_per_slot)) != MPOA_OK) return rc;
+  if ((rc = bin.d_dp.alloc((size_t)n_slots * 3 * caps.dp)) != MPOA_OK) return rc;
+  if (!bin.d_order.p || bin.d_order.n < bin.wins.size())
+    if ((rc = bin.d_order.alloc(std::max<size_t>(bin.wins.size(), 1))) != MPOA_OK) return rc;
+  if (!bin.d_queue.p && (rc = bin.d_queue.alloc(1)) != MPOA_OK) return rc;
+  bin.n_slots = n_slots;
+  bin.used_slots = n_slots;
+  bin.caps = caps;
+  return MPOA_OK;
+}
+
+// Enqueues one persistent launch of a provisioned bin on `stream` (no wait).
+int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
+  if (bin.wins.empty()) return MPOA_OK;
+  int rc;
+  if (!bin.d_order.p || bin.d_order.n < bin.wins.size())
+    if ((rc = bin.d_order.alloc(bin.wins.size())) != MPOA_OK) return rc;
+  if (!bin.d_queue.p && (rc = bin.d_queue.alloc(1)) != MPOA_OK) return rc;
+  HIPCHK(hipMemcpyAsync(bin.d_order.p, bin.wins.data(), bin.wins.size() * sizeof(int),
+                        hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemsetAsync(bin.d_queue.p, 0, sizeof(int), stream));
   KernelArgs a{};
   a.scores = b->sc;
-  a.caps = caps;
+  a.caps = bin.caps;
   a.read_seq = b->d_seq.p;
   a.read_off = b->d_off.p;
   a.win = b->d_win.p;
   a.result = b->d_res.p;
-  a.order = b->d_order.p;
-  a.n_jobs = (int)wins.size();
-  a.queue = b->d_queue.p;
-  a.int_pool = b->d_int.p;
-  a.byte_pool = b->d_byte.p;
-  a.dp_pool = b->d_dp.p;
+  a.order = bin.d_order.p;
+  a.n_jobs = (int)bin.wins.size();
+  a.queue = bin.d_queue.p;
+  a.int_pool = bin.d_int.p;
+  a.byte_pool = bin.d_byte.p;
+  a.dp_pool = bin.d_dp.p;
   a.out = b->d_out.p;
   a.call_stats = b->d_calls.p;
   a.call_stats_stride = b->calls_stride;
   a.call_stats_cap = b->calls_stride;
-  hipEvent_t k0, k1;
-  HIPCHK(hipEventCreate(&k0));
-  HIPCHK(hipEventCreate(&k1));
-  HIPCHK(hipEventRecord(k0, ctx->stream));
-  const int lrc = mpoa_launch_windows(a, n_slots, b->nt, b->cpl, ctx->stream);
+  const int lrc = mpoa_launch_windows(a, bin.used_slots, bin.nt, bin.cpl, stream);
   if (lrc != 0) {
-    (void)hipEventDestroy(k0);
-    (void)hipEventDestroy(k1);
     g_err = lrc == -2 ? "unsupported tile shape" : "kernel launch failed";
     return lrc == -2 ? MPOA_E_ARG : MPOA_E_DEVICE;
   }
-  HIPCHK(hipEventRecord(k1, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  float kms = 0;
-  HIPCHK(hipEventElapsedTime(&kms, k0, k1));
-  (void)hipEventDestroy(k0);
-  (void)hipEventDestroy(k1);
-  b->kernel_ms += kms;
-  if (b->launches == 0) b->first_slots = n_slots;
   b->launches++;
   return MPOA_OK;
 }
@@ -275,6 +282,39 @@ int upload(mpoa_batch* b, int32_t n_windows, const int32_t* win_off, const char*
                         hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return MPOA_OK;
+}
+
+// Splits the windows (kept in processing order) between the small-tile launch (64 threads, a
+// 384-column LDS ring, reads up to 1,536 bp with the query in LDS) and the wide-tile launch
+// (256 threads, a 1,280-column ring) for windows whose band rows, reads or graphs outgrow the
+// small tile's LDS.
+void assign_bins(mpoa_batch* b) {
+  b->small.wins.clear();
+  b->large.wins.clear();
+  b->large.nt = 256;
+  b->large.cpl = 2;
+  const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
+  for (int w : b->order) {
+    const WinInfo& wi = b->info[w];
+    const int m = wi.qmax + 1;
+    int width;
+    if (b->sc.f > 0 && !full) width = 2 * (b->sc.b + m / b->sc.f) + 24;
+    else width = (m / 8 + 2) * 8;
+    const bool large = !b->split || wi.qmax > 1500 || width > 360;
+    (large ? b->large.wins : b->small.wins).push_back(w);
+  }
+  // The costliest remaining windows fill the wide-tile launch up to one window per CU: they
+  // set the makespan, and the wide tile shortens their DP rows and sorts their graphs in LDS.
+  const size_t target = (size_t)std::max(1, b->ctx->n_cu);
+  if (b->split && b->large.wins.size() < target && !b->small.wins.empty()) {
+    const size_t take = std::min(target - b->large.wins.size(), b->small.wins.size() / 8);
+    std::vector<int> moved(b->small.wins.begin(), b->small.wins.begin() + take);
+    b->small.wins.erase(b->small.wins.begin(), b->small.wins.begin() + take);
+    std::vector<int> merged;
+    std::merge(b->large.wins.begin(), b->large.wins.end(), moved.begin(), moved.end(),
+               std::back_inserter(merged), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
+    b->large.wins = merged;
+  }
 }
 
 }  // namespace
@@ -310,7 +350,7 @@ int mpoa_create(int device, mpoa_ctx** out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
-      hipStreamCreate(&c->stream) != hipSuccess) {
+      hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess) {
     delete c;
     g_err = "device initialisation failed";
     return MPOA_E_DEVICE;
@@ -324,6 +364,7 @@ int mpoa_create(int device, mpoa_ctx** out) {
 void mpoa_destroy(mpoa_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 
@@ -380,6 +421,7 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
   for (int w = 0; w < n_windows; w++) b->order[w] = w;
   std::stable_sort(b->order.begin(), b->order.end(),
                    [&](int x, int y) { return b->cost[x] > b->cost[y]; });
+  assign_bins(b);
   int rc;
   if (n_windows > 0 && (rc = upload(b, n_windows, win_off, seq, seq_off)) != MPOA_OK) {
     delete b;
@@ -387,8 +429,6 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
   }
   if ((rc = b->d_win.alloc(std::max(n_windows, 1))) != MPOA_OK ||
       (rc = b->d_res.alloc(std::max(n_windows, 1))) != MPOA_OK ||
-      (rc = b->d_order.alloc(std::max(n_windows, 1))) != MPOA_OK ||
-      (rc = b->d_queue.alloc(1)) != MPOA_OK ||
       (rc = b->d_out.alloc((size_t)std::max<int64_t>(out0, 16))) != MPOA_OK) {
     delete b;
     return rc;
@@ -418,36 +458,79 @@ int mpoa_batch_execute(mpoa_batch* b) {
   std::vector<Estimate> est(n);
   for (int w = 0; w < n; w++) est[w] = estimate(b->info[w], b->sc, full);
 
-  hipEvent_t ev0, ev1;
+  // Device memory: the large-window bin first (few slots), the small-window bin takes the rest.
+  size_t free_b = 0, total_b = 0;
+  int rc;
+  hipEvent_t ev0, ev1, ev_large;
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventCreate(&ev_large));
+  if (!b->large.wins.empty()) {
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const SlotCaps c = make_caps(est, b->info, b->large.wins, keep_path);
+    if ((rc = provision(b->large, c, std::max(1, ctx->n_cu), free_b / 3)) != MPOA_OK) return rc;
+  }
+  if (!b->small.wins.empty()) {
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const SlotCaps c = make_caps(est, b->info, b->small.wins, keep_path);
+    const int max_slots = std::max(1, ctx->n_cu * (b->small.nt == 64 ? 6 : 2));
+    if ((rc = provision(b->small, c, max_slots, free_b / 4 * 3)) != MPOA_OK) return rc;
+  }
+  b->first_slots = b->small.used_slots + b->large.used_slots;
   HIPCHK(hipEventRecord(ev0, ctx->stream));
-  // Pass 1: every window with estimated capacities (largest estimate sets the slot size).
-  const int max_slots = std::max(1, ctx->n_cu * (b->nt == 64 ? 16 : 4));
-  std::vector<int> wins = b->order;
-  int rc = launch(b, wins, make_caps(est, b->info, wins, keep_path), max_slots);
-  if (rc != MPOA_OK) return rc;
+  HIPCHK(hipStreamWaitEvent(ctx->stream2, ev0, 0));
+  // Large windows first so that they hold their CUs from the start.
+  if ((rc = enqueue(b, b->large, ctx->stream2)) != MPOA_OK) return rc;
+  if ((rc = enqueue(b, b->small, ctx->stream)) != MPOA_OK) return rc;
+  HIPCHK(hipEventRecord(ev_large, ctx->stream2));
+  HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
+  HIPCHK(hipEventRecord(ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ev1));
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, ev0, ev1));
+  b->kernel_ms = kms;
   std::vector<WinResult> res(n);
   HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
-  // Later passes: windows that overflowed, with worst-case capacities.
-  for (int pass = 0; pass < 2; pass++) {
+  // Windows that overflowed an estimated capacity: re-run with worst-case capacities.
+  std::vector<int> wins;
+  for (int w = 0; w < n; w++)
+    if (res[w].status == MPOA_E_OVERFLOW) wins.push_back(w);
+  for (int pass = 0; pass < 2 && !wins.empty(); pass++) {
+    b->reruns += (int64_t)wins.size();
+    for (int w : wins) est[w] = worst_case(b->info[w]);
+    Bin& rb = b->rerun;
+    rb.nt = 256;
+    rb.cpl = 2;
+    rb.wins = wins;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    if ((rc = provision(rb, make_caps(est, b->info, wins, keep_path), ctx->n_cu,
+                        free_b / 4 * 3)) != MPOA_OK)
+      return rc;
+    hipEvent_t r0, r1;
+    HIPCHK(hipEventCreate(&r0));
+    HIPCHK(hipEventCreate(&r1));
+    HIPCHK(hipEventRecord(r0, ctx->stream));
+    if ((rc = enqueue(b, rb, ctx->stream)) != MPOA_OK) return rc;
+    HIPCHK(hipEventRecord(r1, ctx->stream));
+    HIPCHK(hipEventSynchronize(r1));
+    float rms = 0;
+    HIPCHK(hipEventElapsedTime(&rms, r0, r1));
+    b->kernel_ms += rms;
+    (void)hipEventDestroy(r0);
+    (void)hipEventDestroy(r1);
+    HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
     std::vector<int> again;
     for (int w : wins)
       if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
-    if (again.empty()) break;
-    b->reruns += (int64_t)again.size();
-    for (int w : again) est[w] = worst_case(b->info[w]);
-    rc = launch(b, again, make_caps(est, b->info, again, keep_path), max_slots);
-    if (rc != MPOA_OK) return rc;
-    HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
     wins = again;
   }
+  float ms = 0;
   HIPCHK(hipEventRecord(ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ev1));
-  float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  (void)hipEventDestroy(ev_large);
   b->device_ms = ms;
   b->res = res;
   for (int w = 0; w < n; w++)
@@ -531,8 +614,8 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
 
 int mpoa_batch_set_tile(mpoa_batch* b, int32_t threads, int32_t cols) {
   if (!b || !mpoa_tile_supported(threads, cols)) return MPOA_E_ARG;
-  b->nt = threads;
-  b->cpl = cols;
+  b->small.nt = threads;
+  b->small.cpl = cols;
   return MPOA_OK;
 }
 
@@ -548,8 +631,8 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   mpoa_batch* b = nullptr;
   int rc = mpoa_batch_prepare(ctx, params, 1, win_off, seq ? seq : "", so, nullptr, &b);
   if (rc != MPOA_OK) return rc;
-  b->nt = 256;
-  b->cpl = 2;
+  b->split = false;  // one window: the wide-tile launch
+  assign_bins(b);
   b->calls_stride = std::max(1, n_reads);
   if ((rc = b->d_calls.alloc((size_t)b->calls_stride)) != MPOA_OK) {
     mpoa_batch_free(b);

@@ -201,6 +201,134 @@ __device__ void add_path(Arena& A, WinShared& S, const SlotCaps& c, bool keep_pa
   add_adj(A, S, c, keep_path, seq_id, anchor, 1, pos, path0);
 }
 
+// add_path over all lanes. A read's path visits each node at most once, so every in-list and
+// out-list touched by one read belongs to a distinct node and the edits are independent; new
+// nodes take ids in res order (a prefix sum), exactly as the serial walk assigns them. A read
+// whose res would create the same aligned node twice falls back to the serial walk. Returns
+// false when that fallback is needed (nothing has been modified then).
+template <int NT>
+__device__ bool add_path_parallel(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path,
+                                  int seq_id, int m, int path0, int* cur) {
+  // 1. claim empty aligned slots (key al*NB+b) for the earliest res entry: atomicMin of
+  //    INT32_MIN + r (every claim is below -1, the empty mark; node ids are >= 0)
+  bool dup = false;
+  for (int r = threadIdx.x; r < m; r += NT) {
+    if (A.res_from[r] < 0 && A.res_aligned[r] >= 0) {
+      const int key = A.res_aligned[r] * NB + A.res_base[r];
+      if (A.aligned[key] < 0) atomicMin(&A.aligned[key], INT32_MIN + r);
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < m; r += NT) {
+    if (A.res_from[r] < 0 && A.res_aligned[r] >= 0) {
+      const int key = A.res_aligned[r] * NB + A.res_base[r];
+      const int v = A.aligned[key];
+      if (v < -1 && v != INT32_MIN + r) dup = true;
+    }
+  }
+  if (__syncthreads_or(dup)) {
+    for (int r = threadIdx.x; r < m; r += NT) {  // undo the claims
+      if (A.res_from[r] < 0 && A.res_aligned[r] >= 0) {
+        const int key = A.res_aligned[r] * NB + A.res_base[r];
+        if (A.aligned[key] < -1) A.aligned[key] = -1;
+      }
+    }
+    __syncthreads();
+    return false;
+  }
+  // 2. node of every res entry; new ids by prefix sum in res order
+  const int n0 = S.n_nodes;
+  int base = 0;
+  for (int r0 = 0; r0 < m; r0 += NT) {
+    const int r = r0 + threadIdx.x;
+    int is_new = 0, v = -1;
+    if (r < m) {
+      v = A.res_from[r];
+      if (v < 0) {
+        const int al = A.res_aligned[r];
+        if (al >= 0) {
+          const int ex = A.aligned[al * NB + A.res_base[r]];
+          if (ex >= 0) v = ex;
+          else is_new = 1;
+        } else {
+          is_new = 1;
+        }
+      }
+    }
+    int tot;
+    const int ex = block_excl_sum<NT>(is_new, S, &tot);
+    if (r < m) {
+      if (is_new) {
+        v = n0 + base + ex;
+        if (v < c.nodes) {
+          const int b = A.res_base[r];
+          A.base[v] = (uint8_t)b;
+          for (int k = 0; k < NB; k++) A.aligned[v * NB + k] = -1;
+          A.aligned[v * NB + b] = v;
+          A.ind[v] = 0;
+          A.in_head[v] = A.in_tail[v] = A.out_head[v] = A.out_tail[v] = -1;
+          A.in_cnt[v] = 0;
+          A.out_cnt[v] = 0;
+          A.last_read[v] = -1;
+          const int al = A.res_aligned[r];
+          if (A.res_from[r] < 0 && al >= 0) {
+            A.aligned[al * NB + b] = v;
+            A.par[v] = al;
+          } else {
+            A.par[v] = v;
+          }
+        }
+      }
+      cur[r] = v;
+    }
+    base += tot;
+  }
+  if (n0 + base > c.nodes) {
+    if (threadIdx.x == 0) S.status = MPOA_E_OVERFLOW;
+    __syncthreads();
+    return true;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) S.n_nodes = n0 + base;
+  // 3. edges cur[r-1] -> cur[r] (r = 1..m-1) and cur[m-1] -> sink, at read position r-1
+  for (int r = 1 + threadIdx.x; r <= m; r += NT) {
+    const int from = cur[r - 1], to = r < m ? cur[r] : 1, pos = r - 1;
+    if (from == to) continue;
+    int e;
+    for (e = A.out_head[from]; e != -1; e = A.oe_next[e])
+      if (A.oe_dst[e] == to) break;
+    if (e != -1) {
+      A.oe_w[e]++;
+    } else {
+      e = atomicAdd(&S.n_out_e, 1);
+      if (e >= c.edges) { S.status = MPOA_E_OVERFLOW; continue; }
+      A.oe_dst[e] = to; A.oe_w[e] = 1; A.oe_next[e] = -1;
+      if (A.out_tail[from] == -1) A.out_head[from] = e; else A.oe_next[A.out_tail[from]] = e;
+      A.out_tail[from] = e;
+      A.out_cnt[from]++;
+    }
+    A.ind[to]++;
+    for (e = A.in_head[to]; e != -1; e = A.ie_next[e])
+      if (A.ie_src[e] == from) break;
+    if (e != -1) {
+      A.ie_w[e]++;
+    } else {
+      e = atomicAdd(&S.n_in_e, 1);
+      if (e >= c.edges) { S.status = MPOA_E_OVERFLOW; continue; }
+      A.ie_src[e] = from; A.ie_w[e] = 1; A.ie_next[e] = -1;
+      if (A.in_tail[to] == -1) A.in_head[to] = e; else A.ie_next[A.in_tail[to]] = e;
+      A.in_tail[to] = e;
+      A.in_cnt[to]++;
+    }
+    if (A.last_read[to] != seq_id) {
+      A.last_read[to] = seq_id;
+      if (keep_path && to != 1) A.path[path0 + pos] = to;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
 // graph::topsort op=0 (src/graph.cpp:59-123), split into:
 //   1. the LIFO Kahn order (stack seeded with the zero in-degree nodes in ascending id
 //      order), one lane: in LDS when the graph's out-adjacency fits (built there in parallel),
@@ -514,6 +642,7 @@ struct Ring {
   int row[RING];
   int bsbe[RING];
   uint8_t q[4 * WR + 64];  // query codes (sentinel first) when the read fits
+  int4 rec[2 * 64];        // row records of the current 64-row chunk
 };
 
 // Orders this workgroup's LDS accesses across a row boundary (no wait on global stores).
@@ -1056,10 +1185,23 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING,
   const int64_t t1 = now();
   if (tid == 0) S.ticks[PH_PLAN] += t1 - t0;
   int64_t cells = 0, ecells = 0;
-  RowRec next = n > 1 ? load_rowrec(A.rowrec, beg + 1) : RowRec{};
   for (int i = 1; i < n; i++) {
-    RowRec rr = next;
-    if (i + 1 < n) next = load_rowrec(A.rowrec, beg + i + 1);
+    // Row records come through LDS, 64 rows at a time: one wait on HBM per chunk instead of
+    // one per row (a load's wait on gfx9 also waits for every store issued before it).
+    if ((i - 1) % 64 == 0) {
+      lds_barrier<NT>();
+      if (tid < 64 && i + tid < n) {
+        const int4* src = reinterpret_cast<const int4*>(A.rowrec + 8LL * (beg + i + tid));
+        R.rec[2 * tid] = src[0];
+        R.rec[2 * tid + 1] = src[1];
+      }
+      lds_barrier<NT>();
+    }
+    RowRec rr;
+    {
+      const int4 x = R.rec[2 * ((i - 1) % 64)], y = R.rec[2 * ((i - 1) % 64) + 1];
+      rr = RowRec{x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+    }
     int bs, be;
     if (adaptive) {
       if (tid == 0) {
@@ -1177,12 +1319,14 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RIN
     }
     __syncthreads();
     if (S.status != MPOA_OK) break;
-    if (tid == 0) {
+    {
       const int64_t ta = now();
-      add_path(A, S, a.caps, keep_path, r, m, path0);
-      S.ticks[PH_ADD] += now() - ta;
+      if (!add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
+        if (tid == 0) add_path(A, S, a.caps, keep_path, r, m, path0);
+      }
+      __syncthreads();
+      if (tid == 0) S.ticks[PH_ADD] += now() - ta;
     }
-    __syncthreads();
     if (S.status == MPOA_OK) {
       const int64_t tb = now();
       if (r + 1 == wd.n_reads) {
