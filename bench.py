@@ -146,6 +146,16 @@ def main() -> int:
         ", ".join(f"{n}={v:.3f}" for n, v in zip(api.PHASES, ph.sum(axis=0))))
     log(f"slowest window {slow}: nodes={ws[slow].nodes} cells={ws[slow].cells} " +
         ", ".join(f"{n}={v * 1e3:.1f}ms" for n, v in zip(api.PHASES, ph[slow])))
+    tb = np.array([w.t_begin for w in ws], dtype=np.float64)
+    te = np.array([w.t_end for w in ws], dtype=np.float64)
+    lb = np.array([w.launch for w in ws])
+    t0d = tb.min()
+    for L in sorted(set(lb.tolist())):
+        sel = lb == L
+        log(f"launch {L}: {int(sel.sum())} windows, first start {(tb[sel].min() - t0d) * tick_s * 1e3:.1f} ms, "
+            f"last start {(tb[sel].max() - t0d) * tick_s * 1e3:.1f} ms, "
+            f"last end {(te[sel].max() - t0d) * tick_s * 1e3:.1f} ms, "
+            f"busy {((te[sel] - tb[sel]).sum()) * tick_s:.2f} window-s")
 
     ms_per_step = wall * 1000.0 / args.steps
     windows_per_s = args.windows * args.steps / wall

@@ -121,9 +121,10 @@ void mpoa_batch_free(mpoa_batch* b);
  * pipeline phase (DP plan, DP rows, backtrack, add_path, topological sort, output), in ticks
  * of the device wall clock (*tick_hz ticks per second). */
 typedef struct mpoa_window_stat {
-  int32_t status, nodes, edges, pad;
+  int32_t status, nodes, edges, launch;  /* launch: 0 small-tile, 1 wide-tile, 2 re-run */
   int64_t cells, edge_cells;
   int64_t ticks[6];
+  int64_t t_begin, t_end;                /* device wall clock at the window's start and end */
 } mpoa_window_stat;
 int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);
 

@@ -61,8 +61,9 @@ class CallStat(ctypes.Structure):
 
 class WindowStat(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
-                ("ticks", ctypes.c_int64 * 6)]
+                ("launch", ctypes.c_int32), ("cells", ctypes.c_int64),
+                ("edge_cells", ctypes.c_int64), ("ticks", ctypes.c_int64 * 6),
+                ("t_begin", ctypes.c_int64), ("t_end", ctypes.c_int64)]
 
 
 class BatchStats(ctypes.Structure):

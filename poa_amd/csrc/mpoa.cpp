@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <thread>
 #include <iterator>
 #include <cmath>
 #include <cstdio>
@@ -155,6 +157,7 @@ struct mpoa_batch {
   DevBuf<WinResult> d_res;
   DevBuf<uint8_t> d_out;
   Bin small, large, rerun;
+  std::vector<int> bin_of;    // launch of each window (0 small, 1 wide, 2 re-run)
   bool split = true;          // large windows in their own concurrent launch
   std::vector<WinResult> res;
   int64_t reruns = 0;
@@ -284,28 +287,35 @@ int upload(mpoa_batch* b, int32_t n_windows, const int32_t* win_off, const char*
   return MPOA_OK;
 }
 
-// Splits the windows (kept in processing order) between the small-tile launch (64 threads, a
-// 384-column LDS ring, reads up to 1,536 bp with the query in LDS) and the wide-tile launch
-// (256 threads, a 1,280-column ring) for windows whose band rows, reads or graphs outgrow the
-// small tile's LDS.
+// Splits the windows (kept in processing order) between two concurrent launches. The small
+// tile (64 threads, 376-column LDS ring, query up to 1,500 bp in LDS, 27 KB of LDS: six
+// workgroups per CU) takes every window it fits. The other launch uses the medium tile (128
+// threads, 512-column ring, 4 KB query, ~38 KB) when all its windows fit that, else the wide
+// tile (256 threads, 1,280-column ring, 24 KB query, ~88 KB). A single window (minipoa's own
+// run) always takes the wide tile: there the per-row latency of one window is the whole time.
 void assign_bins(mpoa_batch* b) {
   b->small.wins.clear();
   b->large.wins.clear();
-  b->large.nt = 256;
-  b->large.cpl = 2;
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
+  bool need_wide = !b->split;
+  auto width_of = [&](const WinInfo& wi) {
+    const int m = wi.qmax + 1;
+    if (b->sc.f > 0 && !full) return 2 * (b->sc.b + m / b->sc.f) + 24;
+    return (m / 8 + 2) * 8;
+  };
   for (int w : b->order) {
     const WinInfo& wi = b->info[w];
-    const int m = wi.qmax + 1;
-    int width;
-    if (b->sc.f > 0 && !full) width = 2 * (b->sc.b + m / b->sc.f) + 24;
-    else width = (m / 8 + 2) * 8;
+    const int width = width_of(wi);
     const bool large = !b->split || wi.qmax > 1500 || width > 360;
+    if (large && (wi.qmax > 4000 || width > 496)) need_wide = true;
     (large ? b->large.wins : b->small.wins).push_back(w);
   }
-  // The costliest remaining windows fill the wide-tile launch up to one window per CU: they
-  // set the makespan, and the wide tile shortens their DP rows and sorts their graphs in LDS.
-  const size_t target = (size_t)std::max(1, b->ctx->n_cu);
+  b->large.nt = need_wide ? 256 : 128;
+  b->large.cpl = need_wide ? 2 : 4;
+  b->bin_of.assign(b->n_windows, 0);
+  // The costliest remaining windows join the other launch up to half a window per CU: they
+  // set the makespan, and the larger tile shortens their DP rows and sorts their graphs in LDS.
+  const size_t target = (size_t)std::max(1, b->ctx->n_cu / 2);
   if (b->split && b->large.wins.size() < target && !b->small.wins.empty()) {
     const size_t take = std::min(target - b->large.wins.size(), b->small.wins.size() / 8);
     std::vector<int> moved(b->small.wins.begin(), b->small.wins.begin() + take);
@@ -315,6 +325,7 @@ void assign_bins(mpoa_batch* b) {
                std::back_inserter(merged), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
     b->large.wins = merged;
   }
+  for (int w : b->large.wins) b->bin_of[w] = 1;
 }
 
 }  // namespace
@@ -479,8 +490,16 @@ int mpoa_batch_execute(mpoa_batch* b) {
   b->first_slots = b->small.used_slots + b->large.used_slots;
   HIPCHK(hipEventRecord(ev0, ctx->stream));
   HIPCHK(hipStreamWaitEvent(ctx->stream2, ev0, 0));
-  // Large windows first so that they hold their CUs from the start.
+  // Large windows first so that they hold their CUs from the start: the small-window grid
+  // (persistent, more workgroups than fit) would otherwise take every CU's LDS and keep the
+  // large windows waiting until its queue drained. The host gives the large launch a short
+  // head start before enqueueing the small one.
   if ((rc = enqueue(b, b->large, ctx->stream2)) != MPOA_OK) return rc;
+  if (!b->large.wins.empty() && !b->small.wins.empty()) {
+    HIPCHK(hipStreamQuery(ctx->stream2) == hipErrorNotReady ? hipSuccess
+                                                              : hipStreamQuery(ctx->stream2));
+    std::this_thread::sleep_for(std::chrono::milliseconds(2));
+  }
   if ((rc = enqueue(b, b->small, ctx->stream)) != MPOA_OK) return rc;
   HIPCHK(hipEventRecord(ev_large, ctx->stream2));
   HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
@@ -502,6 +521,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
     rb.nt = 256;
     rb.cpl = 2;
     rb.wins = wins;
+    for (int w : wins) b->bin_of[w] = 2;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     if ((rc = provision(rb, make_caps(est, b->info, wins, keep_path), ctx->n_cu,
                         free_b / 4 * 3)) != MPOA_OK)
@@ -604,7 +624,9 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
     o.status = r.status;
     o.nodes = r.nodes;
     o.edges = r.edges;
-    o.pad = 0;
+    o.launch = b->bin_of.empty() ? 0 : b->bin_of[w];
+    o.t_begin = r.t_begin;
+    o.t_end = r.t_end;
     o.cells = r.cells;
     o.edge_cells = r.edge_cells;
     for (int k = 0; k < 6; k++) o.ticks[k] = r.ticks[k];

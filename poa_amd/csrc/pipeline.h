@@ -30,10 +30,10 @@ struct SlotCaps {
 
 // Int32 and byte words a slot's arena takes; must match carve() in pipeline.hip.
 inline __host__ __device__ int64_t slot_ints(int N, int E, int qmax, int path) {
-  return 40LL * N + 8LL * E + 2LL * qmax + 8 + path;
+  return 44LL * N + 8LL * E + 2LL * qmax + 8 + path;
 }
 inline __host__ __device__ int64_t slot_bytes(int N, int E, int qmax) {
-  return (int64_t)N + E + 2LL * qmax + 64;
+  return (int64_t)N + E + 3LL * qmax + 128;
 }
 
 struct WinDesc {
@@ -52,6 +52,7 @@ struct WinResult {
   int64_t cells, edge_cells;
   int nodes, edges;
   int64_t ticks[PH_N];
+  int64_t t_begin, t_end;  // device wall clock at the window's start and end
 };
 
 struct CallStat {

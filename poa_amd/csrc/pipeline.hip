@@ -43,6 +43,10 @@ __device__ __forceinline__ uint8_t char_of(int code) {
 }
 
 // Per-slot arena: carved from the int pool / byte pool / DP pool at fixed offsets.
+// Row-record flag: a successor reads the row back from HBM.
+constexpr int RR_FAR = 1 << 12;
+constexpr int RING_ROWS = 8;  // rows kept in the LDS ring (every tile)
+
 struct Arena {
   uint8_t* base;
   int *par, *aligned, *ind, *rank, *in_head, *in_tail, *out_head, *out_tail, *in_cnt, *last_read;
@@ -51,10 +55,10 @@ struct Arena {
   int *rank_id, *t0, *t1, *t2, *hlen, *tlen;
   int *pred_off, *pred_rank, *pred_w, *Bs, *Be, *row_off;
   uint8_t* pred_base;
-  int* rowrec;  // RowRec per rank (8 ints)
+  int* rowrec;  // RowRec per rank (12 ints)
   int *Mp, *Sl, *Pl, *Pr, *Ol, *Or, *Ow;
   int *res_from, *res_aligned;
-  uint8_t *res_base, *qseq;
+  uint8_t *res_base, *qseq, *qsh;
   int* path;
   int *H, *D, *I;
 };
@@ -65,7 +69,7 @@ __device__ Arena carve(const KernelArgs& a, int slot) {
   int* p = a.int_pool + (int64_t)slot * c.ints_per_slot;
   auto take = [&](int64_t n) { int* r = p; p += n; return r; };
   const int N = c.nodes, E = c.edges;
-  A.rowrec = take(8LL * N);  // first: 16-byte aligned for the int4 accesses
+  A.rowrec = take(12LL * N);  // first: 16-byte aligned for the int4 accesses
   A.par = take(N); A.aligned = take((int64_t)NB * N); A.ind = take(N); A.rank = take(N);
   A.in_head = take(N); A.in_tail = take(N); A.out_head = take(N); A.out_tail = take(N);
   A.in_cnt = take(N); A.last_read = take(N); A.out_cnt = take(N);
@@ -83,7 +87,11 @@ __device__ Arena carve(const KernelArgs& a, int slot) {
   A.base = b; b += N;
   A.pred_base = b; b += E;
   A.res_base = b; b += c.qmax + 2;
-  A.qseq = b;
+  auto align16 = [](uint8_t* x) {
+    return reinterpret_cast<uint8_t*>((reinterpret_cast<uintptr_t>(x) + 15) & ~uintptr_t(15));
+  };
+  A.qseq = align16(b); b = A.qseq + c.qmax + 32;
+  A.qsh = align16(b);
   int* dp = a.dp_pool + (int64_t)slot * 3 * c.dp;
   // The int words taken above never exceed slot_ints(N, E, qmax, path) (pipeline.h).
   A.H = dp; A.D = dp + c.dp; A.I = dp + 2 * c.dp;
@@ -107,6 +115,7 @@ struct WinShared {
   int scan[16];
   int row_bs, row_be;
   bool q_lds;
+  int q_cap;
   int64_t ticks[PH_N];
 };
 
@@ -501,33 +510,39 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
   for (int r = threadIdx.x; r < n; r += NT) {
     const int u = A.rank_id[r];
     const int k0 = A.pred_off[r];
-    int k = k0, wmax = -1, max_pre = -1, pr3[3] = {0, 0, 0}, pb = 0;
+    const int ind10 = A.ind[u] / 10;
+    int k = k0, wmax = -1, max_pre = -1, pr3[3] = {0, 0, 0}, w3[3] = {0, 0, 0}, pk = 0;
     for (int e = A.in_head[u]; e != -1; e = A.ie_next[e]) {
       const int src = A.ie_src[e], w = A.ie_w[e], pr = A.rank[src];
       const int b = A.base[src];
       A.pred_rank[k] = pr;
       A.pred_w[k] = w;
-      A.pred_base[k] = (uint8_t)b;
+      A.pred_base[k] = (uint8_t)(b | (w >= ind10 ? 0x80 : 0));
       if (k - k0 < 3) {
         pr3[k - k0] = pr;
-        pb |= b << (8 * (k - k0));
+        w3[k - k0] = w;
+        pk |= b << (3 * (k - k0));
+        if (w >= ind10) pk |= 1 << (9 + (k - k0));
       }
       k++;
       if (w > wmax) { wmax = w; max_pre = pr; }
     }
-    int4* rec = reinterpret_cast<int4*>(A.rowrec + 8LL * r);
-    int4 hi;
-    hi.x = pr3[0]; hi.y = pr3[1]; hi.z = pr3[2]; hi.w = pb;
-    rec[1] = hi;
-    A.rowrec[8LL * r + 2] = k0;
-    A.rowrec[8LL * r + 3] = k - k0;
-    hpar[r] = max_pre;
-    if (with_len) {
-      int omax = -1, max_suc = -1;
-      for (int e = A.out_head[u]; e != -1; e = A.oe_next[e])
-        if (A.oe_w[e] > omax) { omax = A.oe_w[e]; max_suc = A.rank[A.oe_dst[e]]; }
-      tchd[r] = max_suc;
+    int omax = -1, max_suc = -1, far_succ = 0;
+    for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
+      const int sr = A.rank[A.oe_dst[e]];
+      if (sr - r >= RING_ROWS) far_succ = 1;
+      if (A.oe_w[e] > omax) { omax = A.oe_w[e]; max_suc = sr; }
     }
+    if (far_succ) pk |= RR_FAR;
+    int* rec = A.rowrec + 12LL * r;
+    rec[2] = k0;
+    rec[3] = k - k0;
+    reinterpret_cast<int4*>(rec)[1] = make_int4(pr3[0], pr3[1], pr3[2], pk);
+    rec[8] = w3[0];
+    rec[9] = w3[1];
+    rec[10] = w3[2];
+    hpar[r] = max_pre;
+    tchd[r] = max_suc;
   }
   __syncthreads();
   if (with_len) {
@@ -620,29 +635,38 @@ __device__ int consensus(Arena& A, WinShared& S, uint8_t* out, int cap) {
 // Banded DP (src/align.cpp:1357-1514).
 // ---------------------------------------------------------------------------------------
 
-// Per-row DP record (8 int32, one per rank), written by the topological sort (predecessors)
-// and the band plan (band, storage offset); read once per row, one row ahead.
-//   [0] row_off   [1] Bs | Be << 16   [2] k0 (first pred in the pred CSR)   [3] n_pred
-//   [4..6] ranks of the first three predecessors   [7] their bases, one byte each
+// Per-row DP record (12 int32, one per rank), written by the topological sort (predecessors)
+// and the band plan (band, storage offsets); read once per row.
+//   [0] row_off: offset of the row in the DP matrices (traceback words in traceback mode)
+//   [1] Bs | Be << 16      [2] k0: first predecessor in the predecessor CSR   [3] n_pred
+//   [4..6] ranks of the first three predecessors
+//   [7] bits 0-8: their bases (3 bits each); bits 9-11: their in-weight >= ind/10 (the
+//       backtrack's match rule, src/align.cpp:1609); bit 12: the row is read back from HBM
+//       by a successor (one at least RING ranks later)
+//   [8..10] in-weights of the first three predecessors
+//   [11] hd_off: offset of the row's H and D copy in HBM, -1 when it has none
 struct RowRec {
-  int row_off, bsbe, k0, np, p0, p1, p2, pb;
+  int row_off, bsbe, k0, np, p0, p1, p2, pk, w0, w1, w2, hd_off;
 };
+__device__ __forceinline__ RowRec rowrec_of(const int4& a, const int4& b, const int4& c) {
+  return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+}
 __device__ __forceinline__ RowRec load_rowrec(const int* rr, int r) {
-  const int4* v = reinterpret_cast<const int4*>(rr + 8LL * r);
-  const int4 a = v[0], b = v[1];
-  return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  const int4* v = reinterpret_cast<const int4*>(rr + 12LL * r);
+  return rowrec_of(v[0], v[1], v[2]);
 }
 
 // LDS ring of the H and D bands of the last RING rows (row r in slot r % RING, column c of the
 // band at offset c - 8*Bs[r]). Rows wider than WR are not cached.
-template <int RING, int WR>
+template <int RING, int WR, int QCAP>
 struct Ring {
   int H[RING * WR];
   int D[RING * WR];
   int row[RING];
   int bsbe[RING];
-  uint8_t q[4 * WR + 64];  // query codes (sentinel first) when the read fits
-  int4 rec[2 * 64];        // row records of the current 64-row chunk
+  uint8_t q[QCAP];         // shifted query codes when the read fits
+  int4 rec[3 * 32];        // row records of the current 32-row chunk
+  static constexpr int kRing = RING, kWR = WR, kQ = QCAP;
 };
 
 // Orders this workgroup's LDS accesses across a row boundary (no wait on global stores).
@@ -692,63 +716,139 @@ __device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total) {
   return max(excl, before);
 }
 
-// A predecessor row's band and whether its H and D are in the LDS ring (slot p % RING).
-struct PredBand {
-  int bs, be;
-  bool lds;
-};
-
-template <int RING, int WR>
-__device__ __forceinline__ PredBand pred_band(const Arena& A, Ring<RING, WR>& R, int i, int p) {
-  PredBand pb;
-  const int slot = p % RING;
-  if (i - p < RING && R.row[slot] == p) {
-    const int bsbe = R.bsbe[slot];
-    pb.bs = bsbe & 0xffff;
-    pb.be = bsbe >> 16;
-    pb.lds = true;
+// CPL consecutive int32 at base[idx] (idx a multiple of CPL): one aligned vector load.
+template <int CPL, class P>
+__device__ __forceinline__ void load_cols(P base, int idx, int* out) {
+  if constexpr (CPL == 1) {
+    out[0] = base[idx];
+  } else if constexpr (CPL == 2) {
+    const int2 v = *reinterpret_cast<const int2*>(base + idx);
+    out[0] = v.x; out[1] = v.y;
+  } else if constexpr (CPL == 4) {
+    const int4 v = *reinterpret_cast<const int4*>(base + idx);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
   } else {
-    pb.bs = A.Bs[p];
-    pb.be = A.Be[p];
-    pb.lds = false;
+    static_assert(CPL == 8, "CPL");
+    const int4 a = *reinterpret_cast<const int4*>(base + idx);
+    const int4 b = *reinterpret_cast<const int4*>(base + idx + 4);
+    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+    out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
   }
-  return pb;
+}
+template <int CPL, class P>
+__device__ __forceinline__ void store_cols(P base, int idx, const int* v) {
+  if constexpr (CPL == 1) {
+    base[idx] = v[0];
+  } else if constexpr (CPL == 2) {
+    *reinterpret_cast<int2*>(base + idx) = make_int2(v[0], v[1]);
+  } else if constexpr (CPL == 4) {
+    *reinterpret_cast<int4*>(base + idx) = make_int4(v[0], v[1], v[2], v[3]);
+  } else {
+    *reinterpret_cast<int4*>(base + idx) = make_int4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<int4*>(base + idx + 4) = make_int4(v[4], v[5], v[6], v[7]);
+  }
 }
 
-// M/D-source accumulation of one predecessor row into a thread's CPL columns. Hrow/Drow point
-// at the first stored column (8*bs) of the predecessor's band; every index used is
-// non-negative and inside the band.
-template <int CPL, class Ptr>
-__device__ __forceinline__ void accumulate(Ptr Hrow, Ptr Drow, int pofs, int cb, int c0,
+// M/D-source accumulation of one predecessor row into a thread's CPL columns [cb, cb+CPL).
+// Hrow/Drow point at the predecessor band's first stored column (8*Bs[p]); len is the number
+// of int32 readable there. Because every band starts and ends on a multiple of 8 columns, a
+// thread's columns are either all inside the predecessor's band or all outside it: inside,
+// one aligned vector load per matrix; outside, a clamped (in-bounds) load whose values the
+// range masks discard. Column cb's diagonal is H[cb-1], one scalar load.
+template <int CPL, class P>
+__device__ __forceinline__ void accumulate(P Hrow, P Drow, int len, int pofs, int cb, int c0,
                                            int mlo, int mhi, int dlo, int dhi, int pbase,
                                            const int* qc, int match, int mismatch, int o, int e,
                                            int* h, int* d) {
+  const int idx = cb - pofs;
+  const int safe = min(max(idx, 0), len - CPL);
+  int hv[CPL], dv[CPL];
+  load_cols<CPL>(Hrow, safe, hv);
+  load_cols<CPL>(Drow, safe, dv);
+  const int hprev = Hrow[min(max(idx - 1, 0), len - 1)];
+  const int mlo_l = mlo - cb, mhi_l = mhi - cb, dlo_l = dlo - cb, dhi_l = dhi - cb;
 #pragma unroll
   for (int l = 0; l < CPL; l++) {
-    const int c = cb + l;
-    if (c >= mlo && c < mhi) {
-      const int hd = Hrow[max(c - 1 - pofs, 0)];
-      const int diag = c == c0 ? NEG_INF : hd;
-      h[l] = max(h[l], diag + (pbase == qc[l] ? match : mismatch));
+    const int hp = l == 0 ? (cb == c0 ? NEG_INF : hprev) : hv[l - 1];
+    const int cand = hp + (pbase == qc[l] ? match : mismatch);
+    const int mv = max(h[l], cand);
+    h[l] = (l >= mlo_l && l < mhi_l) ? mv : h[l];
+    const int dd = max(d[l], max(dv[l] + e, hv[l] + o));
+    d[l] = (l >= dlo_l && l < dhi_l) ? dd : d[l];
+  }
+}
+
+// CPL query codes for columns [cb, cb+CPL) from the shifted query qs (qs[c] = code of the
+// query character column c consumes; qs[0] = N). cb is a multiple of CPL.
+template <int CPL, class P>
+__device__ __forceinline__ void load_q(P qs, int cb, int cap, int* qc) {
+  const int at = min(cb, cap - CPL);
+  if constexpr (CPL == 1) {
+    qc[0] = qs[at];
+  } else if constexpr (CPL == 2) {
+    const uint32_t v = *reinterpret_cast<const uint16_t*>(qs + at);
+    qc[0] = v & 0xff; qc[1] = v >> 8;
+  } else if constexpr (CPL == 4) {
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(qs + at);
+#pragma unroll
+    for (int l = 0; l < 4; l++) qc[l] = (v >> (8 * l)) & 0xff;
+  } else {
+    const uint2 v = *reinterpret_cast<const uint2*>(qs + at);
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      qc[l] = (v.x >> (8 * l)) & 0xff;
+      qc[4 + l] = (v.y >> (8 * l)) & 0xff;
     }
-    if (c >= dlo && c < dhi) {
-      const int j = max(c - pofs, 0);
-      d[l] = max(d[l], max(Drow[j] + e, Hrow[j] + o));
-    }
+  }
+}
+
+// Waits for this wave's outstanding HBM loads (and stores) only: s_waitcnt vmcnt(0) with the
+// export and LDS counters left at their maxima. Ends every branch that reads HBM inside a DP
+// row, so that at the branch's join no load is pending and the compiler's wait insertion has
+// no reason to wait there, on the LDS-only path, for the previous row's stores.
+__device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// One predecessor's M/D-source contribution to this thread's columns (the band conditions of
+// src/align.cpp:1467 and :1484 as contiguous column ranges), from the LDS ring or from HBM.
+template <int CPL, class RingT>
+__device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpScores& sc, int i,
+                                             int p, int pbase, int cb, int c0, int c1,
+                                             const int* qc, int* h, int* d) {
+  constexpr int RING = RingT::kRing, WR = RingT::kWR;
+  if (p == 0) pbase = CODE_N;
+  const int slot = p % RING;
+  if (i - p < RING && R.row[slot] == p) {
+    const int bsbe = R.bsbe[slot];
+    const int bs = bsbe & 0xffff, be = bsbe >> 16;
+    const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
+    const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
+    accumulate<CPL>(R.H + slot * WR, R.D + slot * WR, WR, bs * Q, cb, c0, mlo, mhi, dlo, dhi,
+                    pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
+  } else {
+    const int bs = A.Bs[p], be = A.Be[p], off = A.row_off[p];
+    const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
+    const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
+    accumulate<CPL>(A.H + off, A.D + off, (be - bs) * Q, bs * Q, cb, c0, mlo, mhi, dlo, dhi,
+                    pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
+    drain_vmem();
   }
 }
 
 // One DP row i (local) for rows in rank window [beg_i, end_i]: M/D sources from every
-// in-window predecessor under the per-block band conditions of src/align.cpp:1467 and :1484
-// (contiguous column ranges), then the I/H scan as an exact prefix max:
+// in-window predecessor, then the I/H scan as an exact prefix max:
 //   I[j] = j*e + max(NEG_INF, max_{k<j} (G[k] + o - (k+1)*e)),  H = max(G, I),
 // G = max(M-source, D-source). This equals the serial recurrence of align.cpp:1507-1514,
 // I[j] = max(I[j-1]+e, H[j-1]+o), because o <= e (gap_open <= 0) and int32 never wraps here.
-// Writes H, D, I (and the NEG_INF sentinel block) to HBM and H, D to the LDS ring.
-// Returns the first argmax column (local) of the row's H, for the adaptive band.
-template <int NT, int CPL, int RING, int WR>
-__device__ int dp_row(const Arena& A, WinShared& S, Ring<RING, WR>& R, const DpScores& sc,
-                      int i, const RowRec& rr, int bs, int be, bool want_argmax) {
+// Writes H, D, I (and the NEG_INF sentinel block) to HBM and H, D to the LDS ring. The common
+// path reads LDS only: on gfx9 a wait for any HBM load also waits for every store issued
+// before it, so HBM reads here (predecessors outside the ring, predecessors after the third,
+// a query too long for LDS) sit in branches of their own.
+// Returns the first argmax column (local) of the row's H, for the adaptive band; *np_in gets
+// the number of in-window predecessors.
+template <int NT, int CPL, bool QLDS, class RingT>
+__device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
+                      const RowRec& rr, int bs, int be, bool want_argmax, int* np_in) {
+  constexpr int RING = RingT::kRing, WR = RingT::kWR;
   const int tid = threadIdx.x;
   const int c0 = bs * Q, W = (be - bs - 1) * Q, c1 = c0 + W;
   int* Hi = A.H + rr.row_off;  // column c at Hi[c - c0]
@@ -760,47 +860,36 @@ __device__ int dp_row(const Arena& A, WinShared& S, Ring<RING, WR>& R, const DpS
   int* RDi = R.D + slot_i * WR;
   const int o = sc.o1, e = sc.e1;
   const int beg = S.beg_i;
+  const int n3 = min(rr.np, 3);
   int carry = SCAN_ID;
   int best = SCAN_ID, best_j = 0x7fffffff;
+  int npi = 0;
   for (int pb = c0; pb < c1; pb += NT * CPL) {
     const int cb = pb + tid * CPL;
     int h[CPL], d[CPL];
 #pragma unroll
     for (int l = 0; l < CPL; l++) h[l] = d[l] = NEG_INF;
     int qc[CPL];
-    const bool q_lds = S.q_lds;
-#pragma unroll
-    for (int l = 0; l < CPL; l++) {
-      const int c = cb + l;
-      const int ci = min(max(c - 1, 0), c1 - 1);
-      const int v = q_lds ? R.q[ci] : A.qseq[ci];
-      qc[l] = (c > 0 && c < c1) ? v : CODE_N;
-    }
-    for (int k = 0; k < rr.np; k++) {
-      int prank, pbase;
-      if (k < 3) {
-        prank = k == 0 ? rr.p0 : k == 1 ? rr.p1 : rr.p2;
-        pbase = (rr.pb >> (8 * k)) & 0xff;
-      } else {
-        prank = A.pred_rank[rr.k0 + k];
-        pbase = A.pred_base[rr.k0 + k];
-      }
+    if constexpr (QLDS) load_q<CPL>(R.q, cb, RingT::kQ, qc);
+    else load_q<CPL>(A.qsh, cb, S.q_cap, qc);
+    npi = 0;
+    for (int k = 0; k < n3; k++) {
+      const int prank = k == 0 ? rr.p0 : k == 1 ? rr.p1 : rr.p2;
       const int p = prank - beg;
       if (p < 0) continue;
-      if (p == 0) pbase = CODE_N;
-      const PredBand pb = pred_band<RING, WR>(A, R, i, p);
-      const int mlo = max(c0, pb.bs == 0 ? 0 : (pb.bs + 1) * Q), mhi = min(c1, (pb.be - 1) * Q);
-      const int dlo = max(c0, pb.bs * Q), dhi = min(c1, (pb.be - 1) * Q);
-      const int pofs = pb.bs * Q;
-      if (pb.lds) {
-        const int slot = p % RING;
-        accumulate<CPL>(R.H + slot * WR, R.D + slot * WR, pofs, cb, c0, mlo, mhi, dlo, dhi,
-                        pbase, qc, sc.match, sc.mismatch, o, e, h, d);
-      } else {
-        const int off = A.row_off[p];
-        accumulate<CPL>(A.H + off, A.D + off, pofs, cb, c0, mlo, mhi, dlo, dhi, pbase, qc,
-                        sc.match, sc.mismatch, o, e, h, d);
+      npi++;
+      pred_contrib<CPL>(A, R, sc, i, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d);
+    }
+    if (rr.np > 3) {
+      for (int k = 3; k < rr.np; k++) {
+        const int p = A.pred_rank[rr.k0 + k] - beg;
+        const int pbase = A.pred_base[rr.k0 + k] & 7;
+        drain_vmem();
+        if (p < 0) continue;
+        npi++;
+        pred_contrib<CPL>(A, R, sc, i, p, pbase, cb, c0, c1, qc, h, d);
       }
+      drain_vmem();
     }
     int g[CPL], t[CPL];
     int lane_max = SCAN_ID;
@@ -824,22 +913,17 @@ __device__ int dp_row(const Arena& A, WinShared& S, Ring<RING, WR>& R, const DpS
         if (want_argmax && hv[l] > best) { best = hv[l]; best_j = j; }
       }
       const int j0 = cb - c0;
-#pragma unroll
-      for (int l = 0; l < CPL; l++) {
-        Hi[j0 + l] = hv[l];
-        Di[j0 + l] = d[l];
-        Ii[j0 + l] = iv[l];
-      }
+      store_cols<CPL>(Hi, j0, hv);
+      store_cols<CPL>(Di, j0, d);
+      store_cols<CPL>(Ii, j0, iv);
       if (cache_i) {
-#pragma unroll
-        for (int l = 0; l < CPL; l++) {
-          RHi[j0 + l] = hv[l];
-          RDi[j0 + l] = d[l];
-        }
+        store_cols<CPL>(RHi, j0, hv);
+        store_cols<CPL>(RDi, j0, d);
       }
     }
     carry = max(carry, total);
   }
+  *np_in = npi;
   // NEG_INF sentinel block after the band (align.cpp:1505-1506,1515).
   if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = NEG_INF;
   if (tid == 0) {
@@ -915,9 +999,10 @@ __device__ bool plan_static(const Arena& A, WinShared& S, const DpScores& sc, co
     if (r < n) {
       const int off = (int)(base + before + v - len);
       A.row_off[r] = off;
-      int* rec = A.rowrec + 8LL * (beg + r);
+      int* rec = A.rowrec + 12LL * (beg + r);
       rec[0] = off;
       rec[1] = A.Bs[r] | (A.Be[r] << 16);
+      rec[11] = off;
     }
     base += all;
   }
@@ -966,7 +1051,7 @@ __device__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc) {
       for (int k = k0; k < k1; k++) {
         const int p = A.pred_rank[k] - beg;
         if (p < 0) continue;
-        const int pbase = p == 0 ? CODE_N : A.pred_base[k];
+        const int pbase = p == 0 ? CODE_N : (A.pred_base[k] & 7);
         if (pbase != qc) continue;
         const int pj = acj - A.Bs[p] * Q;
         const int bn = A.Be[p] - A.Bs[p] - 1;
@@ -1032,7 +1117,7 @@ __device__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc) {
       for (int k = k0; k < k1; k++) {
         const int p = A.pred_rank[k] - beg;
         if (p < 0) continue;
-        const int pbase = p == 0 ? CODE_N : A.pred_base[k];
+        const int pbase = p == 0 ? CODE_N : (A.pred_base[k] & 7);
         const int pj = acj - A.Bs[p] * Q;
         const int bn = A.Be[p] - A.Bs[p] - 1;
         if (pj - 1 >= 0 && pj - 1 < bn * Q &&
@@ -1120,14 +1205,16 @@ __device__ void adaptive_update(const Arena& A, WinShared& S, int i, int max_j, 
 // LDS ring shape per workgroup size: RING rows of WR columns (int32 H and D).
 template <int NT>
 struct RingShape {
-  static constexpr int RING = 8;
-  static constexpr int WR = NT == 64 ? 384 : NT == 128 ? 512 : NT == 256 ? 1280 : 2048;
+  static constexpr int RING = RING_ROWS;
+  static constexpr int WR = NT == 64 ? 368 : NT == 128 ? 512 : NT == 256 ? 1280 : 2048;
+  // longest query held in LDS
+  static constexpr int QCAP = NT == 64 ? 4 * WR + 64 : NT == 128 ? 4096 : 24576;
 };
 
 // One poa() call on the device graph: rows [beg_i, end_i] of the current rank order.
-template <int NT, int CPL, int RING, int WR>
-__device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING, WR>& R,
-                         bool adaptive) {
+template <int NT, int CPL, bool QLDS, class RingT>
+__device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, bool adaptive) {
+  constexpr int RING = RingT::kRing, WR = RingT::kWR;
   const DpScores& sc = a.scores;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int tid = threadIdx.x;
@@ -1186,22 +1273,20 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING,
   if (tid == 0) S.ticks[PH_PLAN] += t1 - t0;
   int64_t cells = 0, ecells = 0;
   for (int i = 1; i < n; i++) {
-    // Row records come through LDS, 64 rows at a time: one wait on HBM per chunk instead of
+    // Row records come through LDS, 32 rows at a time: one wait on HBM per chunk instead of
     // one per row (a load's wait on gfx9 also waits for every store issued before it).
-    if ((i - 1) % 64 == 0) {
+    if ((i - 1) % 32 == 0) {
       lds_barrier<NT>();
-      if (tid < 64 && i + tid < n) {
-        const int4* src = reinterpret_cast<const int4*>(A.rowrec + 8LL * (beg + i + tid));
-        R.rec[2 * tid] = src[0];
-        R.rec[2 * tid + 1] = src[1];
+      if (tid < 32 && i + tid < n) {
+        const int4* src = reinterpret_cast<const int4*>(A.rowrec + 12LL * (beg + i + tid));
+        R.rec[3 * tid] = src[0];
+        R.rec[3 * tid + 1] = src[1];
+        R.rec[3 * tid + 2] = src[2];
       }
       lds_barrier<NT>();
     }
-    RowRec rr;
-    {
-      const int4 x = R.rec[2 * ((i - 1) % 64)], y = R.rec[2 * ((i - 1) % 64) + 1];
-      rr = RowRec{x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
-    }
+    const int ci = (i - 1) % 32;
+    RowRec rr = rowrec_of(R.rec[3 * ci], R.rec[3 * ci + 1], R.rec[3 * ci + 2]);
     int bs, be;
     if (adaptive) {
       if (tid == 0) {
@@ -1224,18 +1309,25 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING,
       return;
     }
     // Predecessors outside the ring are read from HBM: with several waves, their stores must
-    // have completed first.
-    int np_in = 0;
-    bool far = false;
-    for (int k = 0; k < rr.np; k++) {
-      const int prank = k == 0 ? rr.p0 : k == 1 ? rr.p1 : k == 2 ? rr.p2 : A.pred_rank[rr.k0 + k];
-      const int p = prank - beg;
-      if (p < 0) continue;
-      np_in++;
-      far |= !(i - p < RING && R.row[p % RING] == p);
+    // have completed first (one wave sees its own stores in order).
+    if (NT > 64) {
+      bool far = false;
+      const int n3 = min(rr.np, 3);
+      for (int k = 0; k < n3; k++) {
+        const int p = (k == 0 ? rr.p0 : k == 1 ? rr.p1 : rr.p2) - beg;
+        if (p >= 0) far |= !(i - p < RING && R.row[p % RING] == p);
+      }
+      if (rr.np > 3) {
+        for (int k = 3; k < rr.np; k++) {
+          const int p = A.pred_rank[rr.k0 + k] - beg;
+          drain_vmem();
+          if (p >= 0) far |= !(i - p < RING && R.row[p % RING] == p);
+        }
+      }
+      if (far) __syncthreads();
     }
-    if (NT > 64 && far) __syncthreads();
-    const int mj = dp_row<NT, CPL, RING, WR>(A, S, R, sc, i, rr, bs, be, adaptive);
+    int np_in = 0;
+    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in);
     lds_barrier<NT>();
     cells += W;
     ecells += (int64_t)np_in * W;
@@ -1259,13 +1351,14 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING,
 }
 
 // The whole per-window pipeline.
-template <int NT, int CPL, int RING, int WR>
-__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RING, WR>& R, int w) {
+template <int NT, int CPL, class RingT>
+__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, int w) {
   const WinDesc& wd = a.win[w];
   const int tid = threadIdx.x;
   const bool keep_path = a.scores.result != 0;
   const bool with_len = a.scores.f > 0;
   if (tid == 0) {
+    a.result[w].t_begin = now();
     S.status = MPOA_OK;
     S.n_nodes = 0; S.n_in_e = 0; S.n_out_e = 0;
     S.cells = 0; S.edge_cells = 0;
@@ -1286,14 +1379,22 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RIN
     }
     // Query: sentinel N, codes, N padding to col_size (align.cpp:1362-1373).
     const int col_size = (m + 1 + Q - 1) / Q * Q;
-    const bool q_lds = col_size + Q <= (int)sizeof(R.q);
-    for (int j = tid; j < col_size + Q; j += NT) {
+    // qsh / R.q: the same codes shifted one column right (entry c is the character column c
+    // consumes in the M-source, seq[c-1]; entry 0 is N).
+    const bool q_lds = col_size + 2 * Q <= RingT::kQ;
+    for (int j = tid; j < col_size + 2 * Q; j += NT) {
       const uint8_t cq =
           j >= 1 && j <= qlen ? (uint8_t)code_of(a.read_seq[q0 + j - 1]) : (uint8_t)CODE_N;
+      const uint8_t cs =
+          j >= 2 && j <= qlen + 1 ? (uint8_t)code_of(a.read_seq[q0 + j - 2]) : (uint8_t)CODE_N;
       A.qseq[j] = cq;
-      if (q_lds) R.q[j] = cq;
+      A.qsh[j] = cs;
+      if (q_lds) R.q[j] = cs;
     }
-    if (tid == 0) { S.read = r; S.qlen = qlen; S.m = m; S.q_lds = q_lds; }
+    if (tid == 0) {
+      S.read = r; S.qlen = qlen; S.m = m; S.q_lds = q_lds;
+      S.q_cap = col_size + 2 * Q;
+    }
     __syncthreads();
     if (S.n_nodes <= 2) {
       // First read: source then insertions (align.cpp:1747-1753).
@@ -1309,7 +1410,9 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RIN
         S.n_rows = S.end_i - S.beg_i + 1;
       }
       __syncthreads();
-      poa_call<NT, CPL, RING, WR>(a, A, S, R, a.scores.ab_band != 0 && a.scores.f > 0);
+      const bool adaptive = a.scores.ab_band != 0 && a.scores.f > 0;
+      if (S.q_lds) poa_call<NT, CPL, true>(a, A, S, R, adaptive);
+      else poa_call<NT, CPL, false>(a, A, S, R, adaptive);
       if (tid == 0 && a.call_stats && r - 1 < a.call_stats_cap) {
         CallStat& cs = a.call_stats[(int64_t)w * a.call_stats_stride + (r - 1)];
         cs.score = S.score;
@@ -1383,6 +1486,7 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RIN
     S.ticks[PH_OUT] += now() - t_out;
     WinResult& res = a.result[w];
     for (int k = 0; k < PH_N; k++) res.ticks[k] = S.ticks[k];
+    res.t_end = now();
     res.status = S.status;
     res.out_len = S.status == MPOA_OK ? S.cons_len : 0;
     res.cells = S.cells;
@@ -1396,8 +1500,7 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, Ring<RIN
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
   __shared__ WinShared S;
-  constexpr int RING = RingShape<NT>::RING, WR = RingShape<NT>::WR;
-  __shared__ Ring<RING, WR> R;
+  __shared__ Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP> R;
   Arena A = carve(a, blockIdx.x);
   for (;;) {
     if (threadIdx.x == 0) S.job = atomicAdd(a.queue, 1);
@@ -1405,7 +1508,7 @@ __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
     const int job = S.job;
     __syncthreads();
     if (job >= a.n_jobs) break;
-    run_window<NT, CPL, RING, WR>(a, A, S, R, a.order[job]);
+    run_window<NT, CPL>(a, A, S, R, a.order[job]);
   }
 }
 
