@@ -79,6 +79,20 @@ def cpu_baseline(batch, gpu_texts, sample: int, cores: int) -> dict:
     }
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration);
+    None when the summary does not cover the kernel."""
+    path = os.path.join(ROOT, "profiles", "k_windows_uneven10k_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("traffic_bytes_per_launch") and kernel in ln:
+                return int(ln.rsplit(None, 1)[-1])
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +102,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None, help="threads per window workgroup")
     ap.add_argument("--cpl", type=int, default=None, help="DP columns per thread")
-    ap.add_argument("--cpu-sample", type=int, default=320)
+    ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -131,10 +145,13 @@ def main() -> int:
     for i in range(args.warmup):
         log(f"warmup {i}: {b.execute():.1f} ms")
     kernel_ms = []
+    launch_ms: dict[int, list[float]] = {}
     t0 = time.perf_counter()
     for i in range(args.steps):
         b.execute()
         kernel_ms.append(b.stats().kernel_ms)
+        for ls in b.launch_stats():
+            launch_ms.setdefault(ls.launch, []).append(ls.ms)
         log(f"step {i}: kernel {kernel_ms[-1]:.1f} ms")
     wall = time.perf_counter() - t0
     res = b.results()
@@ -161,10 +178,20 @@ def main() -> int:
     windows_per_s = args.windows * args.steps / wall
     gcups = st.cells * args.steps / wall / 1e9
     ebar = st.edge_cells / max(st.cells, 1)
-    bytes_per_cell = 4 * (3 + 2 * ebar)            # int32 H, D, I written; H, D read per edge
-    alg_bytes = st.cells * bytes_per_cell          # per launch
-    kms = float(np.mean(kernel_ms))
-    achieved = alg_bytes / (kms / 1000.0) / 1e9
+    # Dominant kernel: the launch that processes the most DP cells. Algorithmic bytes per cell:
+    # int32 H, D and I written once, H and D of the predecessor row read once per edge-cell.
+    lst = {ls.launch: ls for ls in b.launch_stats()}
+    dom = max(lst.values(), key=lambda ls: ls.cells)
+    dom_ebar = dom.edge_cells / max(dom.cells, 1)
+    bytes_per_cell = 4 * (3 + 2 * dom_ebar)
+    alg_bytes = dom.cells * bytes_per_cell
+    dom_ms = float(np.mean(launch_ms[dom.launch]))
+    achieved = alg_bytes / (dom_ms / 1000.0) / 1e9
+    kname = f"mpoa::k_windows<{dom.threads}, {dom.cols}>"
+    traffic = measured_traffic(kname)
+    for L, ls in sorted(lst.items()):
+        log(f"launch {L} ({ls.threads}x{ls.cols}, {ls.slots} slots): {ls.windows} windows, "
+            f"{ls.cells / 1e9:.2f} G cells, {np.mean(launch_ms[L]):.1f} ms (HIP events)")
     line = {
         "metric": "POA windows/sec on 1 MI355X (bit-exact to minipoa), with GCUPS",
         "value": round(windows_per_s, 1),
@@ -200,10 +227,14 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "mpoa::k_windows",
-            "kernel_ms": round(kms, 3),
+            "traffic": traffic,
+            "kernel": kname,
+            "kernel_ms": round(dom_ms, 3),
+            "windows": int(dom.windows),
+            "band_cells": int(dom.cells),
             "bytes_per_cell": round(bytes_per_cell, 3),
+            "alg_bytes": int(alg_bytes),
+            "all_launches_ms": round(float(np.mean(kernel_ms)), 3),
         },
     }
     if not args.no_cpu:

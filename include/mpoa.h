@@ -128,6 +128,17 @@ typedef struct mpoa_window_stat {
 } mpoa_window_stat;
 int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);
 
+/* Per-launch record of the last execute: tile, resident workgroups, windows and DP cells the
+ * launch processed, and its duration from HIP events recorded on the stream it ran on. */
+typedef struct mpoa_launch_stat {
+  int32_t launch;                        /* 0 small-tile, 1 wide-tile, 2 re-run */
+  int32_t threads, cols, slots;
+  int64_t windows, cells, edge_cells;
+  double ms;
+} mpoa_launch_stat;
+/* Writes up to cap records to out and their number to *n. */
+int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ ERRORS = {
 EXPORTS = [
     "mpoa_create", "mpoa_destroy", "mpoa_strerror", "mpoa_align_reads", "mpoa_batch_prepare",
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
-    "mpoa_batch_window_stats",
+    "mpoa_batch_window_stats", "mpoa_batch_launch_stats",
 ]
 
 PHASES = ("plan", "dp", "backtrack", "add_path", "topsort", "output")
@@ -66,6 +66,12 @@ class WindowStat(ctypes.Structure):
                 ("t_begin", ctypes.c_int64), ("t_end", ctypes.c_int64)]
 
 
+class LaunchStat(ctypes.Structure):
+    _fields_ = [("launch", ctypes.c_int32), ("threads", ctypes.c_int32), ("cols", ctypes.c_int32),
+                ("slots", ctypes.c_int32), ("windows", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("edge_cells", ctypes.c_int64), ("ms", ctypes.c_double)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("windows", ctypes.c_int64), ("dp_calls", ctypes.c_int64),
                 ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
@@ -98,6 +104,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_free.restype = None
         L.mpoa_batch_set_tile.argtypes = [VP, I32, I32]
         L.mpoa_batch_window_stats.argtypes = [VP, VP, I32, P(ctypes.c_double)]
+        L.mpoa_batch_launch_stats.argtypes = [VP, VP, I32, P(I32)]
         _lib = L
     return _lib
 
@@ -205,6 +212,13 @@ class Batch:
         hz = ctypes.c_double()
         _check(lib().mpoa_batch_window_stats(self.h, arr, self.n, ctypes.byref(hz)))
         return [arr[i] for i in range(self.n)], 1.0 / hz.value
+
+    def launch_stats(self) -> list[LaunchStat]:
+        """Per-launch records (tile, windows, cells, HIP-event milliseconds) of the last execute."""
+        arr = (LaunchStat * 8)()
+        n = ctypes.c_int32()
+        _check(lib().mpoa_batch_launch_stats(self.h, arr, 8, ctypes.byref(n)))
+        return [arr[i] for i in range(min(n.value, 8))]
 
     def stats(self) -> BatchStats:
         st = BatchStats()

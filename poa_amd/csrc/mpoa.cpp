@@ -165,6 +165,7 @@ struct mpoa_batch {
   double kernel_ms = 0;
   int launches = 0;
   int first_slots = 0;
+  std::vector<mpoa_launch_stat> lstats;  // launches of the last execute
   // per-call statistics (single-window runs)
   DevBuf<CallStat> d_calls;
   int calls_stride = 0;
@@ -266,6 +267,55 @@ int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
     return lrc == -2 ? MPOA_E_ARG : MPOA_E_DEVICE;
   }
   b->launches++;
+  return MPOA_OK;
+}
+
+// A launch in flight: its bin's windows and the HIP events around it on its stream.
+struct Pending {
+  int kind;
+  const Bin* bin;
+  std::vector<int> wins;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+int timed_enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream, int kind,
+                  std::vector<Pending>& pend) {
+  if (bin.wins.empty()) return MPOA_OK;
+  Pending p;
+  p.kind = kind;
+  p.bin = &bin;
+  p.wins = bin.wins;
+  HIPCHK(hipEventCreate(&p.e0));
+  HIPCHK(hipEventCreate(&p.e1));
+  HIPCHK(hipEventRecord(p.e0, stream));
+  const int rc = enqueue(b, bin, stream);
+  HIPCHK(hipEventRecord(p.e1, stream));
+  pend.push_back(std::move(p));
+  return rc;
+}
+
+// Turns finished launches into records (events synchronised, results copied back).
+int close_pending(mpoa_batch* b, std::vector<Pending>& pend, const std::vector<WinResult>& res) {
+  for (Pending& p : pend) {
+    HIPCHK(hipEventSynchronize(p.e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, p.e0, p.e1));
+    mpoa_launch_stat ls{};
+    ls.launch = p.kind;
+    ls.threads = p.bin->nt;
+    ls.cols = p.bin->cpl;
+    ls.slots = p.bin->used_slots;
+    ls.windows = (int64_t)p.wins.size();
+    for (int w : p.wins) {
+      ls.cells += res[w].cells;
+      ls.edge_cells += res[w].edge_cells;
+    }
+    ls.ms = ms;
+    b->lstats.push_back(ls);
+    (void)hipEventDestroy(p.e0);
+    (void)hipEventDestroy(p.e1);
+  }
+  pend.clear();
   return MPOA_OK;
 }
 
@@ -463,7 +513,9 @@ int mpoa_batch_execute(mpoa_batch* b) {
   b->reruns = 0;
   b->kernel_ms = 0;
   b->launches = 0;
+  b->lstats.clear();
   if (n == 0) return MPOA_OK;
+  std::vector<Pending> pend;
   const bool keep_path = b->params.result != 0;
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
   std::vector<Estimate> est(n);
@@ -494,13 +546,13 @@ int mpoa_batch_execute(mpoa_batch* b) {
   // (persistent, more workgroups than fit) would otherwise take every CU's LDS and keep the
   // large windows waiting until its queue drained. The host gives the large launch a short
   // head start before enqueueing the small one.
-  if ((rc = enqueue(b, b->large, ctx->stream2)) != MPOA_OK) return rc;
+  if ((rc = timed_enqueue(b, b->large, ctx->stream2, 1, pend)) != MPOA_OK) return rc;
   if (!b->large.wins.empty() && !b->small.wins.empty()) {
     HIPCHK(hipStreamQuery(ctx->stream2) == hipErrorNotReady ? hipSuccess
                                                               : hipStreamQuery(ctx->stream2));
     std::this_thread::sleep_for(std::chrono::milliseconds(2));
   }
-  if ((rc = enqueue(b, b->small, ctx->stream)) != MPOA_OK) return rc;
+  if ((rc = timed_enqueue(b, b->small, ctx->stream, 0, pend)) != MPOA_OK) return rc;
   HIPCHK(hipEventRecord(ev_large, ctx->stream2));
   HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
   HIPCHK(hipEventRecord(ev1, ctx->stream));
@@ -510,6 +562,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
   b->kernel_ms = kms;
   std::vector<WinResult> res(n);
   HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+  if ((rc = close_pending(b, pend, res)) != MPOA_OK) return rc;
   // Windows that overflowed an estimated capacity: re-run with worst-case capacities.
   std::vector<int> wins;
   for (int w = 0; w < n; w++)
@@ -530,7 +583,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
     HIPCHK(hipEventCreate(&r0));
     HIPCHK(hipEventCreate(&r1));
     HIPCHK(hipEventRecord(r0, ctx->stream));
-    if ((rc = enqueue(b, rb, ctx->stream)) != MPOA_OK) return rc;
+    if ((rc = timed_enqueue(b, rb, ctx->stream, 2, pend)) != MPOA_OK) return rc;
     HIPCHK(hipEventRecord(r1, ctx->stream));
     HIPCHK(hipEventSynchronize(r1));
     float rms = 0;
@@ -539,6 +592,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
     (void)hipEventDestroy(r0);
     (void)hipEventDestroy(r1);
     HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+    if ((rc = close_pending(b, pend, res)) != MPOA_OK) return rc;
     std::vector<int> again;
     for (int w : wins)
       if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
@@ -631,6 +685,14 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
     o.edge_cells = r.edge_cells;
     for (int k = 0; k < 6; k++) o.ticks[k] = r.ticks[k];
   }
+  return MPOA_OK;
+}
+
+int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n) {
+  if (!b || (cap > 0 && !out)) return MPOA_E_ARG;
+  const int k = (int)b->lstats.size();
+  for (int i = 0; i < k && i < cap; i++) out[i] = b->lstats[i];
+  if (n) *n = k;
   return MPOA_OK;
 }
 
