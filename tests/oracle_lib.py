@@ -107,3 +107,58 @@ def batch(p, win_off, seq, seq_off, threads=0):
         raise RuntimeError(f"oracle error {rc}")
     raw = buf.tobytes()
     return [raw[out_off[i]:out_off[i + 1]].decode() for i in range(nw)]
+
+
+def graph(p, seq: np.ndarray, off: np.ndarray) -> dict:
+    """The graph the oracle builds from all reads (orc_graph), as numpy arrays."""
+    L = lib()
+    seq = np.ascontiguousarray(seq, np.uint8)
+    off = np.ascontiguousarray(off, np.int64)
+    op = oparams(p)
+    nn, ne = ctypes.c_int32(), ctypes.c_int32()
+    n = len(off) - 1
+    none = [None] * 13
+    rc = L.orc_graph(ctypes.byref(op), n, _p(seq), _p(off), ctypes.byref(nn), ctypes.byref(ne),
+                     *none)
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    N, E = nn.value, ne.value
+    g = dict(base=np.zeros(N, np.uint8), in_off=np.zeros(N + 1, np.int32),
+             in_src=np.zeros(E, np.int32), in_w=np.zeros(E, np.int32),
+             out_off=np.zeros(N + 1, np.int32), out_dst=np.zeros(E, np.int32),
+             out_w=np.zeros(E, np.int32), rank_to_id=np.zeros(N, np.int32),
+             hlen=np.zeros(N, np.int32), tlen=np.zeros(N, np.int32), par_id=np.zeros(N, np.int32),
+             aligned=np.zeros(5 * N, np.int32))
+    order = ["base", "in_off", "in_src", "in_w", "out_off", "out_dst", "out_w", "rank_to_id",
+             "hlen", "tlen", "par_id", "aligned"]
+    rc = L.orc_graph(ctypes.byref(op), n, _p(seq), _p(off), ctypes.byref(nn), ctypes.byref(ne),
+                     *[_p(g[k]) for k in order])
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    g["n_nodes"], g["n_edges"] = N, E
+    return g
+
+
+def dp_matrix(p, seq: np.ndarray, off: np.ndarray) -> dict:
+    """H, D, I of the DP that aligns the last read to the graph of the reads before it."""
+    L = lib()
+    seq = np.ascontiguousarray(seq, np.uint8)
+    off = np.ascontiguousarray(off, np.int64)
+    op = oparams(p)
+    n = len(off) - 1
+    nr, tot, score = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+    rc = L.orc_dp_matrix(ctypes.byref(op), n, _p(seq), _p(off), ctypes.byref(nr),
+                         ctypes.byref(tot), None, None, None, None, None, None,
+                         ctypes.byref(score))
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    R, T = nr.value, tot.value
+    m = dict(Bs=np.zeros(R, np.int32), Be=np.zeros(R, np.int32), row_off=np.zeros(R + 1, np.int64),
+             H=np.zeros(T, np.int32), D=np.zeros(T, np.int32), I=np.zeros(T, np.int32))
+    rc = L.orc_dp_matrix(ctypes.byref(op), n, _p(seq), _p(off), ctypes.byref(nr),
+                         ctypes.byref(tot), _p(m["Bs"]), _p(m["Be"]), _p(m["row_off"]),
+                         _p(m["H"]), _p(m["D"]), _p(m["I"]), ctypes.byref(score))
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+    m["n_rows"], m["score"] = R, score.value
+    return m
