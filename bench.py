@@ -134,7 +134,9 @@ def main() -> int:
     batch = synth.config3(seed=seed, n_windows=args.windows)
     ctx = api.Context(0)
     params = api.Params()
+    tp = time.perf_counter()
     b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
+    prepare_s = time.perf_counter() - tp
     if args.nt or args.cpl:
         b.set_tile(args.nt or 64, args.cpl or 4)
 
@@ -154,7 +156,9 @@ def main() -> int:
             launch_ms.setdefault(ls.launch, []).append(ls.ms)
         log(f"step {i}: kernel {kernel_ms[-1]:.1f} ms")
     wall = time.perf_counter() - t0
+    tr = time.perf_counter()
     res = b.results()
+    results_s = time.perf_counter() - tr
     st = res.stats
     ws, tick_s = b.window_stats()
     ph = np.array([[w.ticks[k] for k in range(6)] for w in ws], dtype=np.float64) * tick_s
@@ -220,6 +224,11 @@ def main() -> int:
             "reruns": int(st.reruns),
             "launches": int(st.launches),
             "slots": int(st.slots),
+            "pool_bytes": int(st.pool_bytes),
+            "pcie_inclusive_windows_per_s": round(
+                args.windows / (prepare_s + ms_per_step / 1000.0 + results_s), 1),
+            "prepare_s": round(prepare_s, 3),
+            "results_s": round(results_s, 3),
         },
         "roofline": {
             "bound": "hbm",

@@ -95,6 +95,7 @@ typedef struct mpoa_batch_stats {
                             stream the kernel runs on */
   int32_t launches;      /* pipeline launches in the last execute (1 + re-run passes) */
   int32_t slots;         /* resident workgroups of the first launch */
+  int64_t pool_bytes;    /* device memory held by the per-slot arenas */
 } mpoa_batch_stats;
 
 /* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window

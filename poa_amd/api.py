@@ -77,7 +77,7 @@ class BatchStats(ctypes.Structure):
                 ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
                 ("reruns", ctypes.c_int64), ("device_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int32),
-                ("slots", ctypes.c_int32)]
+                ("slots", ctypes.c_int32), ("pool_bytes", ctypes.c_int64)]
 
 
 _lib = None

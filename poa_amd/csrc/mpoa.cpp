@@ -629,6 +629,10 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
     stats->kernel_ms = b->kernel_ms;
     stats->launches = b->launches;
     stats->slots = b->first_slots;
+    stats->pool_bytes = 0;
+    for (const Bin* bin : {&b->small, &b->large, &b->rerun})
+      stats->pool_bytes += (int64_t)(bin->d_int.n * sizeof(int) + bin->d_byte.n +
+                                     bin->d_dp.n * sizeof(int));
   }
   if (!out_off) return MPOA_OK;
   std::vector<uint8_t> raw((size_t)std::max<int64_t>(b->out_total, 1));
