@@ -79,6 +79,28 @@ def cpu_baseline(batch, gpu_texts, sample: int, cores: int) -> dict:
     }
 
 
+def plain_form(ctx, params, batch, sample: int, gpu_texts, batched_wps: float, log) -> dict:
+    """The plain GPU form (one launch of one workgroup per window, windows one after another)
+    on `sample` evenly spaced windows of the same batch; its rate against the batched one."""
+    from poa_amd import api
+    n = batch.n_windows
+    idx = np.unique(np.linspace(0, n - 1, num=min(sample, n)).round().astype(int))
+    sub = batch.subset(idx)
+    pb = api.Batch(ctx, params, sub.win_off, sub.seq, sub.seq_off)
+    pb.set_mode(api.MODE_PLAIN)
+    t0 = time.perf_counter()
+    pb.execute()
+    wall = time.perf_counter() - t0
+    texts = pb.results().texts
+    pb.free()
+    wps = len(idx) / wall
+    log(f"plain form: {len(idx)} windows in {wall:.2f} s = {wps:.1f} windows/s")
+    return {"value": round(wps, 2), "unit": "windows/s", "windows": int(len(idx)),
+            "sample": "evenly spaced windows of the same batch",
+            "batched_over_plain": round(batched_wps / wps, 2),
+            "outputs_identical": all(texts[i] == gpu_texts[int(w)] for i, w in enumerate(idx))}
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on this workload, from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration);
@@ -105,6 +127,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="schedule by the read-length estimate instead of the trained network")
+    ap.add_argument("--plain-sample", type=int, default=100,
+                    help="windows run in the plain one-window-per-launch form (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (torch), the known-size "
                          "kernel that scales rocprof FETCH_SIZE/WRITE_SIZE to bytes")
@@ -128,7 +154,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    from poa_amd import api, synth
+    from poa_amd import api, sched, synth
 
     seed = synth.SEEDS[3] if args.seed is None else args.seed
     batch = synth.config3(seed=seed, n_windows=args.windows)
@@ -136,6 +162,20 @@ def main() -> int:
     params = api.Params()
     tp = time.perf_counter()
     b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
+    learned = None
+    if not args.no_learned and os.path.exists(sched.MODEL):
+        # Learned scheduling: per-call features -> network on the GPU -> per-window cost,
+        # arena sizes and tile choice. Host feature extraction and inference are part of the
+        # preparation, outside the timed steps.
+        tl = time.perf_counter()
+        model = sched.Model.load()
+        net = sched.Network(0)
+        net.set_weights(model.weights)
+        pred = sched.predict(model, net, params, batch.win_off, batch.seq, batch.seq_off)
+        net.close()
+        b.set_predictions(pred)
+        learned = {"model": os.path.relpath(sched.MODEL, ROOT), "predict_s":
+                   round(time.perf_counter() - tl, 3), "trained_on": model.meta}
     prepare_s = time.perf_counter() - tp
     if args.nt or args.cpl:
         b.set_tile(args.nt or 64, args.cpl or 4)
@@ -246,6 +286,10 @@ def main() -> int:
             "all_launches_ms": round(float(np.mean(kernel_ms)), 3),
         },
     }
+    line["config"]["scheduling"] = learned or "read-length estimate"
+    if args.plain_sample:
+        line["plain_one_window_per_launch"] = plain_form(ctx, params, batch, args.plain_sample,
+                                                         res.texts, windows_per_s, log)
     if not args.no_cpu:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(batch, res.texts, args.cpu_sample, cores)

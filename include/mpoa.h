@@ -59,6 +59,10 @@ typedef struct mpoa_call_stat {
   int32_t score;        /* H[n-1][m]: what minipoa prints as "score:" under -V 1 (align.cpp:1577) */
   int64_t cells;        /* sum over rows 1..n-1 of 8*block_num (align.cpp:1446) */
   int64_t edge_cells;   /* sum over rows of in-window in-degree * 8*block_num */
+  int32_t width;        /* band extent: widest row, 8*block_num */
+  int32_t nodes;        /* graph nodes and edges when the call ran */
+  int32_t edges;
+  int32_t window;       /* window index (batch traces) */
 } mpoa_call_stat;
 
 /* minipoa's whole main loop over one read set: for each read in input order,
@@ -129,6 +133,27 @@ typedef struct mpoa_window_stat {
 } mpoa_window_stat;
 int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);
 
+/* Scheduling predictions, one record of `stride` (>= 4) floats per window: predicted cost
+ * (DP cells summed over the window's calls), DP cells of its costliest call, its widest band
+ * row (8*block_num) and its final graph node count. They order the work queue (costliest
+ * first), choose each window's tile, and size the per-slot graph and DP arenas. They never
+ * change a result: a window that outgrows a predicted capacity stops with an overflow flag on
+ * the device and is re-run with worst-case capacities at the reference band. */
+int mpoa_batch_set_predictions(mpoa_batch* b, const float* pred, int32_t stride);
+
+/* Execution form: MPOA_MODE_BATCHED (default) runs every window in persistent launches with a
+ * device work queue; MPOA_MODE_PLAIN launches one workgroup per window, one window after
+ * another in input order (the comparison form). Results are identical. */
+enum { MPOA_MODE_BATCHED = 0, MPOA_MODE_PLAIN = 1 };
+int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
+
+/* Per-DP-call traces: when enabled before mpoa_batch_execute, every DP call records its
+ * geometry (rows, query length, band extent, cells, graph size) and score. mpoa_batch_traces
+ * copies them in window order (window w's calls are its reads 1..n-1); *n receives the
+ * number of records. */
+int mpoa_batch_set_traces(mpoa_batch* b, int32_t enable);
+int mpoa_batch_traces(mpoa_batch* b, mpoa_call_stat* out, int64_t cap, int64_t* n);
+
 /* Per-launch record of the last execute: tile, resident workgroups, windows and DP cells the
  * launch processed, and its duration from HIP events recorded on the stream it ran on. */
 typedef struct mpoa_launch_stat {
@@ -139,6 +164,44 @@ typedef struct mpoa_launch_stat {
 } mpoa_launch_stat;
 /* Writes up to cap records to out and their number to *n. */
 int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n);
+
+/* ------------------------------------------------------------------------------------
+ * Learned scheduling. minipoa sizes its DP buffer by growing it on demand per thread
+ * (src/mem_alloc_utils.cpp:11-40, used at src/align.cpp:1395-1409) and has no scheduler.
+ * Here a small network predicts each DP call's band extent and cell count; the predictions,
+ * summed per window, go to mpoa_batch_set_predictions. Results never depend on them.
+ * ------------------------------------------------------------------------------------ */
+
+/* Eight features per DP call (window w, read r >= 1), in the order of mpoa_batch_traces:
+ * read length, graph node and edge count estimates, chain coverage and anchor count against
+ * the window's first read (minimizers, k = 11, w = 5), the previous read's static band
+ * extent, the window's error-rate estimate, and the score width in bytes. X receives
+ * 8 floats per call; *n_rows the number of calls (pass X = NULL to size). */
+int mpoa_sched_features(const mpoa_params* params, int32_t n_windows, const int32_t* win_off,
+                        const char* seq, const int64_t* seq_off, float* X, int64_t cap,
+                        int64_t* n_rows);
+
+/* The network: 8 -> 64 -> 64 -> 2 with ReLU, FP16 weights, FP32 accumulation. Parameters
+ * are FP16 bit patterns in the order W1[8][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2]
+ * (mpoa_mlp_n_params() of them). Training keeps FP32 master weights (Adam) on the device;
+ * the loss is the mean squared error over samples and outputs, halved. All arrays below are
+ * host arrays: X has 8 floats per sample, T and Y 2. */
+typedef struct mpoa_mlp mpoa_mlp;
+int mpoa_mlp_create(int device, uint64_t seed, mpoa_mlp** out);
+void mpoa_mlp_destroy(mpoa_mlp* m);
+int32_t mpoa_mlp_n_params(void);
+int mpoa_mlp_get_weights(mpoa_mlp* m, uint16_t* out, int32_t cap);
+int mpoa_mlp_set_weights(mpoa_mlp* m, const uint16_t* in, int32_t n);
+int mpoa_mlp_forward(mpoa_mlp* m, const float* X, int64_t n, float* Y);
+/* Gradient of the loss over the n samples with respect to every parameter (FP32, same
+ * order as the weights), and the loss itself; the weights are not changed. */
+int mpoa_mlp_gradients(mpoa_mlp* m, const float* X, const float* T, int64_t n, float* grad,
+                       float* loss);
+/* Adam (lr, beta 0.9/0.999) over `epochs` shuffled passes of minibatches of `batch`
+ * samples; epoch_loss (optional) receives the mean minibatch loss of each epoch. */
+int mpoa_mlp_train(mpoa_mlp* m, const float* X, const float* T, int64_t n, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, float* epoch_loss);
+const char* mpoa_mlp_error(void);
 
 #ifdef __cplusplus
 }

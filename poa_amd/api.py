@@ -26,8 +26,11 @@ ERRORS = {
 EXPORTS = [
     "mpoa_create", "mpoa_destroy", "mpoa_strerror", "mpoa_align_reads", "mpoa_batch_prepare",
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
-    "mpoa_batch_window_stats", "mpoa_batch_launch_stats",
+    "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
+    "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_predictions",
 ]
+
+MODE_BATCHED, MODE_PLAIN = 0, 1
 
 PHASES = ("plan", "dp", "backtrack", "add_path", "topsort", "output")
 
@@ -56,7 +59,8 @@ class Params(ctypes.Structure):
 class CallStat(ctypes.Structure):
     _fields_ = [("read", ctypes.c_int32), ("rows", ctypes.c_int32), ("qlen", ctypes.c_int32),
                 ("score", ctypes.c_int32), ("cells", ctypes.c_int64),
-                ("edge_cells", ctypes.c_int64)]
+                ("edge_cells", ctypes.c_int64), ("width", ctypes.c_int32),
+                ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32), ("window", ctypes.c_int32)]
 
 
 class WindowStat(ctypes.Structure):
@@ -105,6 +109,10 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_set_tile.argtypes = [VP, I32, I32]
         L.mpoa_batch_window_stats.argtypes = [VP, VP, I32, P(ctypes.c_double)]
         L.mpoa_batch_launch_stats.argtypes = [VP, VP, I32, P(I32)]
+        L.mpoa_batch_set_traces.argtypes = [VP, I32]
+        L.mpoa_batch_set_mode.argtypes = [VP, I32]
+        L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
+        L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
         _lib = L
     return _lib
 
@@ -212,6 +220,30 @@ class Batch:
         hz = ctypes.c_double()
         _check(lib().mpoa_batch_window_stats(self.h, arr, self.n, ctypes.byref(hz)))
         return [arr[i] for i in range(self.n)], 1.0 / hz.value
+
+    def set_mode(self, mode: int) -> None:
+        """MODE_BATCHED (persistent launches, work queue) or MODE_PLAIN (one launch per
+        window, in input order)."""
+        _check(lib().mpoa_batch_set_mode(self.h, mode))
+
+    def set_predictions(self, pred: np.ndarray) -> None:
+        """Per-window (cost, max call cells, max band width, final nodes) predictions that
+        order the queue and size the arenas; results never depend on them."""
+        pred = np.ascontiguousarray(pred, np.float32)
+        assert pred.shape == (self.n, 4)
+        _check(lib().mpoa_batch_set_predictions(self.h, _ptr(pred), 4))
+
+    def set_traces(self, enable: bool = True) -> None:
+        """Record every DP call's geometry on the next execute (see traces())."""
+        _check(lib().mpoa_batch_set_traces(self.h, 1 if enable else 0))
+
+    def traces(self) -> np.ndarray:
+        """Per-DP-call records of the last execute as a structured array (window order)."""
+        n = ctypes.c_int64()
+        _check(lib().mpoa_batch_traces(self.h, None, 0, ctypes.byref(n)))
+        arr = (CallStat * max(n.value, 1))()
+        _check(lib().mpoa_batch_traces(self.h, arr, n.value, ctypes.byref(n)))
+        return np.ctypeslib.as_array(arr)[:n.value].copy()
 
     def launch_stats(self) -> list[LaunchStat]:
         """Per-launch records (tile, windows, cells, HIP-event milliseconds) of the last execute."""

@@ -23,13 +23,14 @@ REF_DIR = os.environ.get("MPOA_REF_DIR", "/root/reference")
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-    # Packed FP32 instructions stay off (they are not needed by this integer code).
+    # Packed FP32 instructions stay off (the DP is integer code; the scheduling network's
+    # float code runs unpacked).
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-PRODUCT_SOURCES = ["pipeline.hip", "mpoa.cpp"]
+PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "sched.cpp"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

@@ -124,6 +124,21 @@ Estimate worst_case(const WinInfo& w) {
   return e;
 }
 
+mpoa_call_stat call_stat_of(const CallStat& c, int read, int window) {
+  mpoa_call_stat o{};
+  o.read = read;
+  o.rows = c.rows;
+  o.qlen = c.qlen;
+  o.score = c.score;
+  o.cells = c.cells;
+  o.edge_cells = c.edge_cells;
+  o.width = c.width;
+  o.nodes = c.nodes;
+  o.edges = c.edges;
+  o.window = window;
+  return o;
+}
+
 }  // namespace
 
 // One persistent launch configuration: a set of windows, a tile shape and slot capacities.
@@ -166,9 +181,14 @@ struct mpoa_batch {
   int launches = 0;
   int first_slots = 0;
   std::vector<mpoa_launch_stat> lstats;  // launches of the last execute
+  int mode = MPOA_MODE_BATCHED;
+  // Learned predictions per window (mpoa_batch_set_predictions): DP cells of the costliest
+  // call, widest band row and final node count. Empty when none were given.
+  std::vector<float> pred_cells, pred_width, pred_nodes;
   // per-call statistics (single-window runs)
   DevBuf<CallStat> d_calls;
-  int calls_stride = 0;
+  DevBuf<int64_t> d_call_off;
+  std::vector<int64_t> call_off;  // per window: first call record
 };
 
 namespace {
@@ -259,8 +279,7 @@ int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
   a.dp_pool = bin.d_dp.p;
   a.out = b->d_out.p;
   a.call_stats = b->d_calls.p;
-  a.call_stats_stride = b->calls_stride;
-  a.call_stats_cap = b->calls_stride;
+  a.call_off = b->d_call_off.p;
   const int lrc = mpoa_launch_windows(a, bin.used_slots, bin.nt, bin.cpl, stream);
   if (lrc != 0) {
     g_err = lrc == -2 ? "unsupported tile shape" : "kernel launch failed";
@@ -355,7 +374,7 @@ void assign_bins(mpoa_batch* b) {
   };
   for (int w : b->order) {
     const WinInfo& wi = b->info[w];
-    const int width = width_of(wi);
+    const int width = b->pred_width.empty() ? width_of(wi) : (int)b->pred_width[w];
     const bool large = !b->split || wi.qmax > 1500 || width > 360;
     if (large && (wi.qmax > 4000 || width > 496)) need_wide = true;
     (large ? b->large.wins : b->small.wins).push_back(w);
@@ -376,6 +395,118 @@ void assign_bins(mpoa_batch* b) {
     b->large.wins = merged;
   }
   for (int w : b->large.wins) b->bin_of[w] = 1;
+}
+
+// Slot capacities for window w: the learned predictions when given (the caller includes its
+// safety margin; a window that outgrows them overflows and is re-run with worst-case
+// capacities), else the read-length estimate.
+Estimate capacity_of(const mpoa_batch* b, int w, bool full) {
+  Estimate e = estimate(b->info[w], b->sc, full);
+  if (b->pred_cells.empty()) return e;
+  const WinInfo& wi = b->info[w];
+  const int64_t nodes = std::min<int64_t>(
+      std::max<int64_t>((int64_t)b->pred_nodes[w] + 16, wi.qmax + 2), 2 + wi.qtot);
+  e.nodes = (int)nodes;
+  e.edges = (int)std::min<int64_t>(3 * nodes + wi.n_reads, wi.qtot + wi.n_reads + 2);
+  // storage of one call: its cells, one sentinel block per row and row 0
+  e.dp = (int64_t)b->pred_cells[w] + 8 * nodes + (int64_t)(wi.qmax / 8 + 2) * 8 + 64;
+  if (full) e.dp = std::max(e.dp, estimate(wi, b->sc, full).dp);
+  return e;
+}
+
+// The plain form: one launch per window, one workgroup, one window after another on one
+// stream, in input order. Same kernel and capacities as the batched form.
+int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vector<int>& wins,
+               bool worst, float* ms_out);
+
+int execute_plain(mpoa_batch* b, std::vector<Estimate>& est) {
+  const int n = b->n_windows;
+  std::vector<int> all(n);
+  for (int w = 0; w < n; w++) all[w] = w;
+  float ms = 0, rms = 0;
+  int rc = plain_pass(b, est, all, false, &ms);
+  if (rc != MPOA_OK) return rc;
+  b->kernel_ms = ms;
+  std::vector<WinResult> res(n);
+  HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+  std::vector<int> again;
+  for (int w = 0; w < n; w++)
+    if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
+  if (!again.empty()) {
+    b->reruns += (int64_t)again.size();
+    for (int w : again) est[w] = worst_case(b->info[w]);
+    if ((rc = plain_pass(b, est, again, true, &rms)) != MPOA_OK) return rc;
+    b->kernel_ms += rms;
+    HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+  }
+  b->device_ms = b->kernel_ms;
+  b->res = res;
+  b->first_slots = 1;
+  for (int w = 0; w < n; w++)
+    if (res[w].status != MPOA_OK) return res[w].status;
+  return MPOA_OK;
+}
+
+int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vector<int>& wins,
+               bool worst, float* ms_out) {
+  mpoa_ctx* ctx = b->ctx;
+  const int n = b->n_windows;
+  const bool keep_path = b->params.result != 0;
+  int rc;
+  size_t free_b = 0, total_b = 0;
+  // Worst-case re-runs take the wide tile (the re-run bin of the batched form).
+  if (worst) {
+    b->rerun.nt = 256;
+    b->rerun.cpl = 2;
+    b->rerun.wins = wins;
+  }
+  for (Bin* bin : {&b->small, &b->large, &b->rerun}) {
+    if (bin->wins.empty() || (worst != (bin == &b->rerun))) continue;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    if ((rc = provision(*bin, make_caps(est, b->info, bin->wins, keep_path), 1, free_b / 3)) !=
+        MPOA_OK)
+      return rc;
+  }
+  std::vector<int> ident(n);
+  for (int w = 0; w < n; w++) ident[w] = w;
+  DevBuf<int> d_ident, d_q;
+  if ((rc = d_ident.alloc(n)) != MPOA_OK || (rc = d_q.alloc(n)) != MPOA_OK) return rc;
+  HIPCHK(hipMemcpy(d_ident.p, ident.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(d_q.p, 0, n * sizeof(int)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  for (int w : wins) {
+    const Bin& bin = worst ? b->rerun : b->bin_of[w] == 1 ? b->large : b->small;
+    KernelArgs a{};
+    a.scores = b->sc;
+    a.caps = bin.caps;
+    a.read_seq = b->d_seq.p;
+    a.read_off = b->d_off.p;
+    a.win = b->d_win.p;
+    a.result = b->d_res.p;
+    a.order = d_ident.p + w;
+    a.n_jobs = 1;
+    a.queue = d_q.p + w;
+    a.int_pool = bin.d_int.p;
+    a.byte_pool = bin.d_byte.p;
+    a.dp_pool = bin.d_dp.p;
+    a.out = b->d_out.p;
+    a.call_stats = b->d_calls.p;
+    a.call_off = b->d_call_off.p;
+    if (mpoa_launch_windows(a, 1, bin.nt, bin.cpl, ctx->stream) != 0) {
+      g_err = "kernel launch failed";
+      return MPOA_E_DEVICE;
+    }
+    b->launches++;
+  }
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms_out, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return MPOA_OK;
 }
 
 }  // namespace
@@ -519,7 +650,8 @@ int mpoa_batch_execute(mpoa_batch* b) {
   const bool keep_path = b->params.result != 0;
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
   std::vector<Estimate> est(n);
-  for (int w = 0; w < n; w++) est[w] = estimate(b->info[w], b->sc, full);
+  for (int w = 0; w < n; w++) est[w] = capacity_of(b, w, full);
+  if (b->mode == MPOA_MODE_PLAIN) return execute_plain(b, est);
 
   // Device memory: the large-window bin first (few slots), the small-window bin takes the rest.
   size_t free_b = 0, total_b = 0;
@@ -692,6 +824,68 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
   return MPOA_OK;
 }
 
+int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode) {
+  if (!b || (mode != MPOA_MODE_BATCHED && mode != MPOA_MODE_PLAIN)) return MPOA_E_ARG;
+  b->mode = mode;
+  return MPOA_OK;
+}
+
+int mpoa_batch_set_predictions(mpoa_batch* b, const float* pred, int32_t stride) {
+  if (!b || stride < 4 || (b->n_windows > 0 && !pred)) return MPOA_E_ARG;
+  const int n = b->n_windows;
+  b->pred_cells.assign(n, 0.f);
+  b->pred_width.assign(n, 0.f);
+  b->pred_nodes.assign(n, 0.f);
+  for (int w = 0; w < n; w++) {
+    const float* p = pred + (int64_t)w * stride;
+    if (!(p[0] >= 0.f) || !(p[1] >= 0.f) || !(p[2] >= 0.f) || !(p[3] >= 0.f)) {
+      g_err = "predictions must be finite and non-negative";
+      return MPOA_E_ARG;
+    }
+    b->cost[w] = p[0];
+    b->pred_cells[w] = p[1];
+    b->pred_width[w] = p[2];
+    b->pred_nodes[w] = p[3];
+  }
+  for (int w = 0; w < n; w++) b->order[w] = w;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return b->cost[x] > b->cost[y]; });
+  assign_bins(b);
+  return MPOA_OK;
+}
+
+int mpoa_batch_set_traces(mpoa_batch* b, int32_t enable) {
+  if (!b) return MPOA_E_ARG;
+  if (!enable) {
+    b->d_calls.release();
+    b->d_call_off.release();
+    b->call_off.clear();
+    return MPOA_OK;
+  }
+  const int n = b->n_windows;
+  b->call_off.assign(n + 1, 0);
+  for (int w = 0; w < n; w++) b->call_off[w + 1] = b->call_off[w] + std::max(0, b->info[w].n_reads - 1);
+  int rc;
+  if ((rc = b->d_calls.alloc((size_t)std::max<int64_t>(b->call_off[n], 1))) != MPOA_OK) return rc;
+  if ((rc = b->d_call_off.alloc((size_t)n + 1)) != MPOA_OK) return rc;
+  HIPCHK(hipMemcpy(b->d_call_off.p, b->call_off.data(), (n + 1) * sizeof(int64_t),
+                   hipMemcpyHostToDevice));
+  return MPOA_OK;
+}
+
+int mpoa_batch_traces(mpoa_batch* b, mpoa_call_stat* out, int64_t cap, int64_t* n) {
+  if (!b || (cap > 0 && !out)) return MPOA_E_ARG;
+  const int64_t total = b->call_off.empty() ? 0 : b->call_off.back();
+  if (n) *n = total;
+  if (!out || total == 0) return MPOA_OK;
+  std::vector<CallStat> cs((size_t)total);
+  HIPCHK(hipMemcpy(cs.data(), b->d_calls.p, cs.size() * sizeof(CallStat), hipMemcpyDeviceToHost));
+  for (int w = 0; w < b->n_windows; w++)
+    for (int64_t k = b->call_off[w]; k < b->call_off[w + 1] && k < cap; k++)
+      out[k] = call_stat_of(cs[k], (int)(k - b->call_off[w]) + 1, w);
+  return MPOA_OK;
+}
+
 int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n) {
   if (!b || (cap > 0 && !out)) return MPOA_E_ARG;
   const int k = (int)b->lstats.size();
@@ -721,8 +915,7 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   if (rc != MPOA_OK) return rc;
   b->split = false;  // one window: the wide-tile launch
   assign_bins(b);
-  b->calls_stride = std::max(1, n_reads);
-  if ((rc = b->d_calls.alloc((size_t)b->calls_stride)) != MPOA_OK) {
+  if ((rc = mpoa_batch_set_traces(b, 1)) != MPOA_OK) {
     mpoa_batch_free(b);
     return rc;
   }
@@ -734,12 +927,7 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
     if (hipMemcpy(cs.data(), b->d_calls.p, cs.size() * sizeof(CallStat), hipMemcpyDeviceToHost) ==
         hipSuccess) {
       for (int i = 0; i < (int)cs.size() && i < stats_cap; i++) {
-        stats[i].read = i + 1;
-        stats[i].rows = cs[i].rows;
-        stats[i].qlen = cs[i].qlen;
-        stats[i].score = cs[i].score;
-        stats[i].cells = 0;
-        stats[i].edge_cells = 0;
+        stats[i] = call_stat_of(cs[i], i + 1, 0);
       }
     }
   }

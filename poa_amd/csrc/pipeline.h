@@ -55,8 +55,11 @@ struct WinResult {
   int64_t t_begin, t_end;  // device wall clock at the window's start and end
 };
 
+// One DP call (one read against the graph of the reads before it).
 struct CallStat {
-  int score, rows, qlen, pad;
+  int score, rows, qlen, width;  // width: widest row, 8*block_num
+  int64_t cells, edge_cells;
+  int nodes, edges;              // graph size when the call ran
 };
 
 struct KernelArgs {
@@ -73,9 +76,8 @@ struct KernelArgs {
   uint8_t* byte_pool;
   int* dp_pool;
   uint8_t* out;
-  CallStat* call_stats;     // optional: per window, per DP call
-  int call_stats_stride;
-  int call_stats_cap;
+  CallStat* call_stats;     // optional: per DP call, window w's calls from call_off[w]
+  const int64_t* call_off;  // n_windows + 1 offsets (reads - 1 per window)
 };
 
 }  // namespace mpoa

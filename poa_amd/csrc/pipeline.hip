@@ -110,6 +110,8 @@ struct WinShared {
   int cons_len;
   int msa_cols;
   int64_t cells, edge_cells;
+  int64_t call_cells, call_ecells;  // of the last DP call
+  int call_width;                   // widest row of the last DP call
   int64_t total;  // DP storage of the current call
   int job;
   int scan[16];
@@ -1272,6 +1274,7 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
   const int64_t t1 = now();
   if (tid == 0) S.ticks[PH_PLAN] += t1 - t0;
   int64_t cells = 0, ecells = 0;
+  int wmax = 0;
   for (int i = 1; i < n; i++) {
     // Row records come through LDS, 32 rows at a time: one wait on HBM per chunk instead of
     // one per row (a load's wait on gfx9 also waits for every store issued before it).
@@ -1331,6 +1334,7 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
     lds_barrier<NT>();
     cells += W;
     ecells += (int64_t)np_in * W;
+    wmax = max(wmax, W);
     if (adaptive) {
       if (tid == 0) adaptive_update(A, S, i, mj, w);
       lds_barrier<NT>();
@@ -1342,6 +1346,9 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
     S.ticks[PH_DP] += t2 - t1;
     S.cells += cells;
     S.edge_cells += ecells;
+    S.call_cells = cells;
+    S.call_ecells = ecells;
+    S.call_width = wmax;
     S.score = A.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)];
     const int rc = backtrack(A, S, sc);
     if (rc != MPOA_OK) S.status = rc;
@@ -1413,11 +1420,16 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R
       const bool adaptive = a.scores.ab_band != 0 && a.scores.f > 0;
       if (S.q_lds) poa_call<NT, CPL, true>(a, A, S, R, adaptive);
       else poa_call<NT, CPL, false>(a, A, S, R, adaptive);
-      if (tid == 0 && a.call_stats && r - 1 < a.call_stats_cap) {
-        CallStat& cs = a.call_stats[(int64_t)w * a.call_stats_stride + (r - 1)];
+      if (tid == 0 && a.call_stats && a.call_off[w] + r - 1 < a.call_off[w + 1]) {
+        CallStat& cs = a.call_stats[a.call_off[w] + r - 1];
         cs.score = S.score;
         cs.rows = S.n_rows;
         cs.qlen = qlen;
+        cs.width = S.call_width;
+        cs.cells = S.call_cells;
+        cs.edge_cells = S.call_ecells;
+        cs.nodes = S.n_nodes;
+        cs.edges = S.n_in_e;
       }
     }
     __syncthreads();
