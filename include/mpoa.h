@@ -165,6 +165,14 @@ typedef struct mpoa_launch_stat {
 /* Writes up to cap records to out and their number to *n. */
 int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n);
 
+/* -S seeding on the host: the anchors of read rid of a window on the consensus cons (ASCII,
+ * cons_len bases), as the sub-window split consumes them (minimizer.cpp:537-570 ->
+ * lchain.cpp:158-386); out receives (x, y) pairs: x = consensus position of the anchor's last
+ * base, y = span << 32 | read position (bit 40: band-mode mark). *n = anchor count. */
+int mpoa_seed_anchors(const mpoa_params* params, int32_t n_reads, const char* seq,
+                      const int64_t* seq_off, int32_t rid, const char* cons, int32_t cons_len,
+                      uint64_t* out, int64_t cap, int64_t* n);
+
 /* ------------------------------------------------------------------------------------
  * Learned scheduling. minipoa sizes its DP buffer by growing it on demand per thread
  * (src/mem_alloc_utils.cpp:11-40, used at src/align.cpp:1395-1409) and has no scheduler.

@@ -1453,3 +1453,24 @@ extern "C" int orc_dp_matrix(const orc_params* p, int32_t n_reads, const char* s
   if (I) std::copy(dump.I.begin(), dump.I.end(), I);
   return rc;
 }
+
+// The global chain's anchors of read `rid` against a given consensus (codes or ASCII), as the
+// -S dispatcher consumes them: out[2*i] = x, out[2*i+1] = y.
+extern "C" int orc_anchors(const orc_params* p, int32_t n_reads, const char* seq,
+                           const int64_t* seq_off, int32_t rid, const char* cons,
+                           int32_t cons_len, uint64_t* out, int64_t cap, int64_t* n) {
+  if (!p || rid < 0 || rid >= n_reads) return ORC_E_ARG;
+  Params P = normalise(*p);
+  Seeder S;
+  S.init(P, n_reads, seq, seq_off);
+  std::string c(cons, cons + cons_len);
+  const int qlen = (int)(seq_off[rid + 1] - seq_off[rid]);
+  std::vector<mm128> a = S.anchors_bycons(P, rid, qlen, c);
+  if (n) *n = (int64_t)a.size();
+  for (size_t i = 0; i < a.size() && (int64_t)i < cap; i++) {
+    out[2 * i] = a[i].x;
+    out[2 * i + 1] = a[i].y;
+  }
+  return ORC_OK;
+}
+

@@ -98,6 +98,12 @@ int orc_dp_matrix(const orc_params* p, int32_t n_reads, const char* seq, const i
                   int32_t* n_rows, int64_t* total, int32_t* Bs, int32_t* Be, int64_t* row_off,
                   int32_t* H, int32_t* D, int32_t* I, int32_t* score);
 
+/* Anchors of read rid on the consensus cons (ASCII): the global chain as alignment()'s -S
+ * branch consumes it (x, y pairs into out). */
+int orc_anchors(const orc_params* p, int32_t n_reads, const char* seq, const int64_t* seq_off,
+                int32_t rid, const char* cons, int32_t cons_len, uint64_t* out, int64_t cap,
+                int64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

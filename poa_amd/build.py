@@ -30,7 +30,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "sched.cpp"]
+PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "sched.cpp", "seed.cpp"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

@@ -14,6 +14,7 @@
 
 #include "mpoa.h"
 #include "pipeline.h"
+#include "seed.h"
 
 using mpoa::CallStat;
 using mpoa::DpScores;
@@ -21,6 +22,13 @@ using mpoa::KernelArgs;
 using mpoa::SlotCaps;
 using mpoa::WinDesc;
 using mpoa::WinResult;
+using mpoa::SeedArgs;
+using mpoa::SegDesc;
+using mpoa::SegResult;
+using mpoa::WinState;
+using mpoa::slot_bytes;
+using mpoa::slot_ints;
+namespace seed = mpoa::seed;
 
 struct mpoa_ctx {
   int device = 0;
@@ -42,6 +50,17 @@ thread_local std::string g_err;
       return e_ == hipErrorOutOfMemory ? MPOA_E_OOM : MPOA_E_DEVICE;                      \
     }                                                                                    \
   } while (0)
+
+// nt4 encoding (src/sequence.cpp:6-23): A C G T/U -> 0..3, anything else 4.
+inline uint8_t code_of_char(unsigned char c) {
+  switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': case 'U': case 'u': return 3;
+    default: return 4;
+  }
+}
 
 // initPara (src/handle_input.cpp:45-77) + o1 = O + E (src/align.cpp:1369).
 DpScores scores_of(const mpoa_params& p) {
@@ -509,6 +528,233 @@ int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vecto
   return MPOA_OK;
 }
 
+// A seeded (-S) window, read by read (main.cpp:110-121 with alignment()'s -S branch,
+// align.cpp:1763-1827). The host seeds each read against the consensus the device built
+// after the previous read, splits the read at the chain's anchors, and launches one
+// workgroup per sub-window, then the commit of the read. The graph stays on the device.
+int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
+               std::vector<mpoa_call_stat>* calls) {
+  mpoa_ctx* ctx = b->ctx;
+  const WinInfo& wi = b->info[0];
+  const int n = wi.n_reads;
+  const bool keep_path = b->params.result != 0;
+  const int64_t base0 = seq_off[0];
+  std::vector<uint8_t> codes((size_t)(seq_off[n] - base0) + 1);
+  std::vector<int64_t> off(n + 1);
+  for (int i = 0; i <= n; i++) off[i] = seq_off[i] - base0;
+  for (int64_t j = 0; j < off[n]; j++) codes[j] = code_of_char((unsigned char)seq[base0 + j]);
+  seed::Config cfg;
+  cfg.k = b->params.k;
+  cfg.w = b->params.mm_w;
+  cfg.bw = 1000;  // main.cpp:65
+  cfg.poa_w = b->params.poa_w;
+  seed::Seeder sd;
+  sd.init(cfg, n, codes.data(), off.data());
+
+  DevBuf<WinState> d_state;
+  DevBuf<SegDesc> d_segs;
+  DevBuf<SegResult> d_segres;
+  DevBuf<int> d_splice, d_cons;
+  DevBuf<int> g_int, s_int, s_dp, g_dp;
+  DevBuf<uint8_t> g_byte, s_byte;
+  int rc;
+  if ((rc = d_state.alloc(1)) != MPOA_OK) return rc;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    calls->clear();
+    const Estimate e = attempt == 0 ? estimate(wi, b->sc, true) : worst_case(wi);
+    SlotCaps gc{};
+    gc.nodes = (std::max(e.nodes, 2) + 63) / 64 * 64;
+    gc.edges = (std::max(e.edges, 4) + 63) / 64 * 64;
+    gc.qmax = std::max(1, wi.qmax);
+    gc.path = keep_path ? (int)wi.qtot : 0;
+    gc.dp = 64;
+    gc.ints_per_slot = (slot_ints(gc.nodes, gc.edges, gc.qmax, gc.path) + 63) / 64 * 64;
+    gc.bytes_per_slot = (slot_bytes(gc.nodes, gc.edges, gc.qmax) + 255) / 256 * 256;
+    if ((rc = g_int.alloc((size_t)gc.ints_per_slot)) != MPOA_OK ||
+        (rc = g_byte.alloc((size_t)gc.bytes_per_slot)) != MPOA_OK ||
+        (rc = g_dp.alloc(3 * 64)) != MPOA_OK || (rc = d_cons.alloc(3 * (size_t)gc.nodes)) != MPOA_OK)
+      return rc;
+    KernelArgs ga{};
+    ga.scores = b->sc;
+    ga.caps = gc;
+    ga.read_seq = b->d_seq.p;
+    ga.read_off = b->d_off.p;
+    ga.win = b->d_win.p;
+    ga.result = b->d_res.p;
+    ga.n_jobs = 1;
+    ga.int_pool = g_int.p;
+    ga.byte_pool = g_byte.p;
+    ga.dp_pool = g_dp.p;
+    ga.out = b->d_out.p;
+    SlotCaps sc_caps{};
+    size_t seg_slots = 0;
+    int64_t seg_dp = 0;
+    int path0 = 0, prev_seg = 0;
+    WinState st{};
+    bool failed = false;
+    std::vector<int> cons;
+    std::vector<uint8_t> cons_code;
+    auto collect = [&](int read, int nseg) -> int {
+      if (nseg == 0) return MPOA_OK;
+      std::vector<SegResult> sr(nseg);
+      HIPCHK(hipMemcpy(sr.data(), d_segres.p, nseg * sizeof(SegResult), hipMemcpyDeviceToHost));
+      for (const SegResult& g : sr) {
+        mpoa_call_stat c{};
+        c.read = read;
+        c.rows = g.rows;
+        c.qlen = g.qlen;
+        c.score = g.score;
+        c.cells = g.cells;
+        c.edge_cells = g.edge_cells;
+        c.width = g.width;
+        c.nodes = st.n_nodes;
+        c.edges = st.n_in_e;
+        calls->push_back(c);
+      }
+      return MPOA_OK;
+    };
+    for (int r = 0; r < n && !failed; r++) {
+      const int qlen = (int)(off[r + 1] - off[r]);
+      std::vector<SegDesc> segs;
+      std::vector<int> splice;
+      std::vector<int64_t> need;
+      if (r > 0) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(&st, d_state.p, sizeof(WinState), hipMemcpyDeviceToHost));
+        if ((rc = collect(r - 1, prev_seg)) != MPOA_OK) return rc;
+        if (st.status != MPOA_OK) {
+          failed = true;
+          break;
+        }
+        if (st.n_nodes > 2) {
+          const int L = st.cons_len;
+          cons.resize(3 * (size_t)std::max(L, 1));
+          if (L > 0)
+            HIPCHK(hipMemcpy(cons.data(), d_cons.p, 3 * (size_t)L * sizeof(int),
+                             hipMemcpyDeviceToHost));
+          cons_code.resize(std::max(L, 1));
+          for (int p = 0; p < L; p++) cons_code[p] = (uint8_t)cons[3 * p + 2];
+          const std::vector<seed::Pair> an = sd.anchors(r, qlen, cons_code.data(), L);
+          const int sink_rank = st.n_nodes - 1;
+          int beg_id = 0, beg_rank = 0, beg_q = 0, j = 0, res_pos = 0;
+          auto add_seg = [&](int end_id, int end_rank, int sub_len) {
+            SegDesc d{};
+            d.beg_id = beg_id;
+            d.end_id = end_id;
+            d.q0 = j;
+            d.qlen = sub_len;
+            d.res0 = res_pos;
+            segs.push_back(d);
+            const int rows = end_rank - beg_rank + 1;
+            need.push_back((int64_t)std::max(rows, 1) * ((sub_len + 1) / 8 + 2) * 8);
+            return rows;
+          };
+          for (const seed::Pair& a : an) {
+            const int q_span = (int)(a.y >> 32 & 0xff);
+            const int end_tpos = (int)(uint32_t)a.x - q_span + 1;
+            const int end_qpos = (int)(uint32_t)a.y - q_span + 1;
+            if (end_tpos < 0 || end_tpos + q_span > L) {
+              g_err = "anchor outside the consensus";
+              return MPOA_E_ARG;
+            }
+            const int sub_len = end_qpos - beg_q;
+            const int end_id = cons[3 * end_tpos], end_rank = cons[3 * end_tpos + 1];
+            if (add_seg(end_id, end_rank, sub_len) <= 0) return MPOA_E_EMPTY_WINDOW;
+            res_pos += sub_len + 1;
+            j += sub_len;
+            int last_id = end_id, last_rank = end_rank;
+            for (int k = 0; k < q_span; k++, j++) {
+              last_id = cons[3 * (end_tpos + k)];
+              last_rank = cons[3 * (end_tpos + k) + 1];
+              if (k != q_span - 1) {
+                splice.push_back(res_pos++);
+                splice.push_back(last_id);
+              }
+            }
+            beg_id = last_id;
+            beg_rank = last_rank;
+            beg_q = j;
+          }
+          if (add_seg(1, sink_rank, qlen - beg_q) <= 0) return MPOA_E_EMPTY_WINDOW;
+        }
+      }
+      const int nseg = (int)segs.size();
+      if (nseg > 0) {
+        const int64_t dp_need = (*std::max_element(need.begin(), need.end()) + 63) / 64 * 64;
+        if ((size_t)nseg > seg_slots || dp_need > seg_dp) {
+          sc_caps = gc;
+          sc_caps.edges = 64;
+          sc_caps.path = 0;
+          sc_caps.dp = std::max<int64_t>(dp_need, seg_dp);
+          sc_caps.ints_per_slot =
+              (slot_ints(sc_caps.nodes, sc_caps.edges, sc_caps.qmax, 0) + 63) / 64 * 64;
+          sc_caps.bytes_per_slot = (slot_bytes(sc_caps.nodes, sc_caps.edges, sc_caps.qmax) + 255) /
+                                   256 * 256;
+          const size_t slots = std::max<size_t>((size_t)nseg, seg_slots);
+          HIPCHK(hipStreamSynchronize(ctx->stream));
+          if ((rc = s_int.alloc(slots * sc_caps.ints_per_slot)) != MPOA_OK ||
+              (rc = s_byte.alloc(slots * sc_caps.bytes_per_slot)) != MPOA_OK ||
+              (rc = s_dp.alloc(slots * 3 * (size_t)sc_caps.dp)) != MPOA_OK ||
+              (rc = d_segres.alloc(slots)) != MPOA_OK)
+            return rc;
+          seg_slots = slots;
+          seg_dp = sc_caps.dp;
+        }
+        if (!d_segs.p || d_segs.n < (size_t)nseg)
+          if ((rc = d_segs.alloc(std::max<size_t>(nseg, 16))) != MPOA_OK) return rc;
+        HIPCHK(hipMemcpyAsync(d_segs.p, segs.data(), nseg * sizeof(SegDesc),
+                              hipMemcpyHostToDevice, ctx->stream));
+        if (!splice.empty()) {
+          if (!d_splice.p || d_splice.n < splice.size())
+            if ((rc = d_splice.alloc(std::max<size_t>(splice.size(), 64))) != MPOA_OK) return rc;
+          HIPCHK(hipMemcpyAsync(d_splice.p, splice.data(), splice.size() * sizeof(int),
+                                hipMemcpyHostToDevice, ctx->stream));
+        }
+      }
+      SeedArgs sa{};
+      sa.graph = ga;
+      sa.seg = ga;
+      sa.seg.caps = sc_caps;
+      sa.seg.int_pool = s_int.p;
+      sa.seg.byte_pool = s_byte.p;
+      sa.seg.dp_pool = s_dp.p;
+      sa.segs = d_segs.p;
+      sa.seg_res = d_segres.p;
+      sa.n_seg = nseg;
+      sa.splice = d_splice.p;
+      sa.n_splice = (int)splice.size() / 2;
+      sa.read = r;
+      sa.n_reads = n;
+      sa.path0 = path0;
+      sa.state = d_state.p;
+      sa.cons_out = d_cons.p;
+      sa.cons_cap = gc.nodes;
+      if (nseg > 0 && mpoa_launch_segments(sa, 128, 4, ctx->stream) != 0) {
+        g_err = "sub-window launch failed";
+        return MPOA_E_DEVICE;
+      }
+      if (mpoa_launch_commit(sa, ctx->stream) != 0) {
+        g_err = "commit launch failed";
+        return MPOA_E_DEVICE;
+      }
+      b->launches += nseg > 0 ? 2 : 1;
+      prev_seg = nseg;
+      path0 += qlen;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&st, d_state.p, sizeof(WinState), hipMemcpyDeviceToHost));
+    if (!failed && (rc = collect(n - 1, prev_seg)) != MPOA_OK) return rc;
+    b->res.assign(1, WinResult{});
+    HIPCHK(hipMemcpy(b->res.data(), b->d_res.p, sizeof(WinResult), hipMemcpyDeviceToHost));
+    if (b->res[0].status == MPOA_E_OVERFLOW && attempt == 0) {
+      b->reruns++;
+      continue;
+    }
+    return b->res[0].status;
+  }
+  return b->res.empty() ? MPOA_E_DEVICE : b->res[0].status;
+}
+
 }  // namespace
 
 extern "C" {
@@ -886,6 +1132,35 @@ int mpoa_batch_traces(mpoa_batch* b, mpoa_call_stat* out, int64_t cap, int64_t* 
   return MPOA_OK;
 }
 
+int mpoa_seed_anchors(const mpoa_params* params, int32_t n_reads, const char* seq,
+                      const int64_t* seq_off, int32_t rid, const char* cons, int32_t cons_len,
+                      uint64_t* out, int64_t cap, int64_t* n) {
+  if (!params || !seq || !seq_off || rid < 0 || rid >= n_reads || cons_len < 0) return MPOA_E_ARG;
+  const int64_t base0 = seq_off[0];
+  std::vector<uint8_t> codes((size_t)(seq_off[n_reads] - base0) + 1);
+  std::vector<int64_t> off(n_reads + 1);
+  for (int i = 0; i <= n_reads; i++) off[i] = seq_off[i] - base0;
+  for (int64_t j = 0; j < off[n_reads]; j++)
+    codes[j] = code_of_char((unsigned char)seq[base0 + j]);
+  std::vector<uint8_t> cc((size_t)cons_len + 1);
+  for (int j = 0; j < cons_len; j++) cc[j] = code_of_char((unsigned char)cons[j]);
+  seed::Config cfg;
+  cfg.k = params->k;
+  cfg.w = params->mm_w;
+  cfg.bw = 1000;
+  cfg.poa_w = params->poa_w;
+  seed::Seeder sd;
+  sd.init(cfg, n_reads, codes.data(), off.data());
+  const int qlen = (int)(off[rid + 1] - off[rid]);
+  const std::vector<seed::Pair> a = sd.anchors(rid, qlen, cc.data(), cons_len);
+  if (n) *n = (int64_t)a.size();
+  for (size_t i = 0; i < a.size() && (int64_t)i < cap; i++) {
+    out[2 * i] = a[i].x;
+    out[2 * i + 1] = a[i].y;
+  }
+  return MPOA_OK;
+}
+
 int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n) {
   if (!b || (cap > 0 && !out)) return MPOA_E_ARG;
   const int k = (int)b->lstats.size();
@@ -907,34 +1182,48 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
                      int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
                      int32_t stats_cap, int32_t* n_stats) {
   if (!ctx || !params || n_reads < 0 || (n_reads > 0 && (!seq || !seq_off))) return MPOA_E_ARG;
+  if (params->result == 2) {
+    g_err = "GFA output (-r 2) is not supported";
+    return MPOA_E_ARG;
+  }
   int32_t win_off[2] = {0, n_reads};
   std::vector<int64_t> zero_off{0};
   const int64_t* so = n_reads > 0 ? seq_off : zero_off.data();
+  // The batch carries the reads, the window descriptor and the output buffer; a seeded window
+  // is then driven read by read (run_seeded), any other in one launch.
+  mpoa_params pp = *params;
+  const bool seeding = pp.seeding != 0 && n_reads > 0;
+  pp.seeding = 0;
   mpoa_batch* b = nullptr;
-  int rc = mpoa_batch_prepare(ctx, params, 1, win_off, seq ? seq : "", so, nullptr, &b);
+  int rc = mpoa_batch_prepare(ctx, &pp, 1, win_off, seq ? seq : "", so, nullptr, &b);
   if (rc != MPOA_OK) return rc;
-  b->split = false;  // one window: the wide-tile launch
-  assign_bins(b);
-  if ((rc = mpoa_batch_set_traces(b, 1)) != MPOA_OK) {
-    mpoa_batch_free(b);
-    return rc;
-  }
-  rc = mpoa_batch_execute(b);
-  const WinResult r = b->res.empty() ? WinResult{} : b->res[0];
-  if (n_stats) *n_stats = std::max(0, n_reads - 1);
-  if (stats && n_reads > 1) {
-    std::vector<CallStat> cs(n_reads - 1);
-    if (hipMemcpy(cs.data(), b->d_calls.p, cs.size() * sizeof(CallStat), hipMemcpyDeviceToHost) ==
-        hipSuccess) {
-      for (int i = 0; i < (int)cs.size() && i < stats_cap; i++) {
-        stats[i] = call_stat_of(cs[i], i + 1, 0);
-      }
+  b->params.seeding = params->seeding;
+  std::vector<mpoa_call_stat> calls;
+  if (seeding) {
+    rc = run_seeded(b, seq, so, &calls);
+  } else {
+    b->split = false;  // one window: the wide-tile launch
+    assign_bins(b);
+    if ((rc = mpoa_batch_set_traces(b, 1)) != MPOA_OK) {
+      mpoa_batch_free(b);
+      return rc;
+    }
+    rc = mpoa_batch_execute(b);
+    if (n_reads > 1) {
+      std::vector<CallStat> cs(n_reads - 1);
+      if (hipMemcpy(cs.data(), b->d_calls.p, cs.size() * sizeof(CallStat),
+                    hipMemcpyDeviceToHost) == hipSuccess)
+        for (int i = 0; i < (int)cs.size(); i++) calls.push_back(call_stat_of(cs[i], i + 1, 0));
     }
   }
+  if (n_stats) *n_stats = (int32_t)calls.size();
+  if (stats)
+    for (int i = 0; i < (int)calls.size() && i < stats_cap; i++) stats[i] = calls[i];
   if (rc != MPOA_OK) {
     mpoa_batch_free(b);
     return rc;
   }
+  const WinResult r = b->res.empty() ? WinResult{} : b->res[0];
   std::string text;
   if (params->result == 0) {
     std::vector<char> cons((size_t)std::max(r.out_len, 1));

@@ -30,10 +30,10 @@ struct SlotCaps {
 
 // Int32 and byte words a slot's arena takes; must match carve() in pipeline.hip.
 inline __host__ __device__ int64_t slot_ints(int N, int E, int qmax, int path) {
-  return 44LL * N + 8LL * E + 2LL * qmax + 8 + path;
+  return 47LL * N + 8LL * E + 2LL * qmax + 8 + path;
 }
 inline __host__ __device__ int64_t slot_bytes(int N, int E, int qmax) {
-  return (int64_t)N + E + 3LL * qmax + 128;
+  return 2LL * N + E + 3LL * qmax + 128;
 }
 
 struct WinDesc {
@@ -80,6 +80,40 @@ struct KernelArgs {
   const int64_t* call_off;  // n_windows + 1 offsets (reads - 1 per window)
 };
 
+// -S sub-window of the current read: the DP of read characters [q0, q0 + qlen) against the
+// graph's rank window [rank(beg_id), rank(end_id)]; its alignment goes to res [res0, ...).
+struct SegDesc {
+  int beg_id, end_id, q0, qlen, res0, pad;
+};
+
+struct SegResult {
+  int status, score, rows, qlen, width, pad;
+  int64_t cells, edge_cells;
+  int64_t ticks[PH_N];
+};
+
+// Window state kept in device memory between the launches of a seeded window.
+struct WinState {
+  int n_nodes, n_in_e, n_out_e, status, msa_cols, cons_len, pad0, pad1;
+  int64_t cells, edge_cells;
+  int64_t ticks[PH_N];
+};
+
+struct SeedArgs {
+  KernelArgs graph;        // the window (win[0]) and the graph in slot 0 of its pools
+  KernelArgs seg;          // private scratch per sub-window (slot = sub-window)
+  const SegDesc* segs;
+  SegResult* seg_res;
+  int n_seg;               // 0: trivial read (graph without bases)
+  const int* splice;       // (res position, node id) pairs: anchor bases
+  int n_splice;
+  int read, n_reads;
+  int path0;               // bases of the reads before this one
+  WinState* state;
+  int* cons_out;           // after a commit: (node id, rank, base) per consensus position
+  int cons_cap;
+};
+
 }  // namespace mpoa
 
 // Launches the persistent pipeline with n_slots workgroups of nt threads, cpl DP columns per
@@ -87,3 +121,7 @@ struct KernelArgs {
 int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
                         hipStream_t stream);
 bool mpoa_tile_supported(int nt, int cpl);
+// Seeded windows: every sub-window of one read (one workgroup each), then the commit of that
+// read (one workgroup). 0, -1 on a launch error, -2 for an unsupported tile.
+int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream);
+int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream);

@@ -58,7 +58,9 @@ struct Arena {
   int* rowrec;  // RowRec per rank (12 ints)
   int *Mp, *Sl, *Pl, *Pr, *Ol, *Or, *Ow;
   int *res_from, *res_aligned;
+  int *cons_id, *cons_rank, *cons_pre;  // consensus path for -S seeding
   uint8_t *res_base, *qseq, *qsh;
+  uint8_t* cons_ch;
   int* path;
   int *H, *D, *I;
 };
@@ -83,8 +85,10 @@ __device__ Arena carve(const KernelArgs& a, int slot) {
   A.Ol = take(N); A.Or = take(N); A.Ow = take(N);
   A.res_from = take(c.qmax + 2); A.res_aligned = take(c.qmax + 2);
   A.path = take(c.path);
+  A.cons_id = take(N); A.cons_rank = take(N); A.cons_pre = take(N);
   uint8_t* b = a.byte_pool + (int64_t)slot * c.bytes_per_slot;
   A.base = b; b += N;
+  A.cons_ch = b; b += N;
   A.pred_base = b; b += E;
   A.res_base = b; b += c.qmax + 2;
   auto align16 = [](uint8_t* x) {
@@ -1357,106 +1361,125 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
   __syncthreads();
 }
 
-// The whole per-window pipeline.
-template <int NT, int CPL, class RingT>
-__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, int w) {
-  const WinDesc& wd = a.win[w];
+// Query slice src[0..qlen): sentinel N, the codes, N padding to col_size
+// (align.cpp:1362-1373); plain in A.qseq and shifted one column right in A.qsh and, when it
+// fits, R.q (entry c is the character column c consumes in the M-source, seq[c-1]).
+template <int NT, class RingT>
+__device__ void load_query(Arena& A, WinShared& S, RingT& R, const uint8_t* src, int qlen) {
+  const int tid = threadIdx.x;
+  const int m = qlen + 1;
+  const int col_size = (m + 1 + Q - 1) / Q * Q;
+  const bool q_lds = col_size + 2 * Q <= RingT::kQ;
+  for (int j = tid; j < col_size + 2 * Q; j += NT) {
+    const uint8_t cq = j >= 1 && j <= qlen ? (uint8_t)code_of(src[j - 1]) : (uint8_t)CODE_N;
+    const uint8_t cs = j >= 2 && j <= qlen + 1 ? (uint8_t)code_of(src[j - 2]) : (uint8_t)CODE_N;
+    A.qseq[j] = cq;
+    A.qsh[j] = cs;
+    if (q_lds) R.q[j] = cs;
+  }
+  if (tid == 0) {
+    S.qlen = qlen;
+    S.m = m;
+    S.q_lds = q_lds;
+    S.q_cap = col_size + 2 * Q;
+  }
+  __syncthreads();
+}
+
+// First read: the source, then one insertion per base (align.cpp:1747-1753).
+template <int NT>
+__device__ void trivial_res(Arena& A, int m) {
+  for (int j = threadIdx.x; j < m; j += NT) {
+    A.res_from[j] = j == 0 ? 0 : -1;
+    A.res_base[j] = j == 0 ? (uint8_t)CODE_N : A.qseq[j];
+    A.res_aligned[j] = -1;
+  }
+  __syncthreads();
+}
+
+// graph::build_consensus (graph.cpp:294-350) into the consensus arrays: FIFO BFS from the
+// source, each node keeping the predecessor of largest in-weight (ties to the larger
+// accumulated score); the path back from the sink, in order, with each node's rank.
+// One lane. Returns the consensus length.
+__device__ int build_consensus_path(Arena& A, WinShared& S) {
+  const int N = S.n_nodes;
+  int* deg = A.t0;
+  int* queue = A.t1;
+  int* score = A.t2;
+  int* pre = A.cons_pre;
+  for (int v = 0; v < N; v++) {
+    deg[v] = A.in_cnt[v];
+    score[v] = 0;
+    pre[v] = 0;
+  }
+  int qh = 0, qt = 0;
+  queue[qt++] = 0;
+  while (qh < qt) {
+    const int u = queue[qh++];
+    if (u != 0) {
+      int wmax = -1, max_pre = -1;
+      for (int e = A.in_head[u]; e != -1; e = A.ie_next[e]) {
+        const int v = A.ie_src[e], w = A.ie_w[e];
+        if (w > wmax || (w == wmax && score[v] > score[max_pre])) {
+          wmax = w;
+          max_pre = v;
+        }
+      }
+      pre[u] = max_pre;
+      score[u] = score[max_pre] + wmax;
+    }
+    if (u == 1) break;
+    for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
+      const int v = A.oe_dst[e];
+      if (--deg[v] == 0) queue[qt++] = v;
+    }
+  }
+  int len = 0;
+  for (int id = pre[1]; id != 0; id = pre[id]) len++;
+  int at = len;
+  for (int id = pre[1]; id != 0; id = pre[id]) {
+    --at;
+    A.cons_id[at] = id;
+    A.cons_ch[at] = A.base[id];
+    A.cons_rank[at] = A.rank[id];
+  }
+  return len;
+}
+
+// add_path then the re-sort of one read (main.cpp:118-120): op=1 after the last read, else
+// op=0 (with hlen/tlen for the static or adaptive band) and, when seeding, the consensus
+// the next read is seeded against. lds / lds_bytes: LDS scratch for the sort.
+template <int NT>
+__device__ void commit_read(const KernelArgs& a, Arena& A, WinShared& S, int r, int m,
+                            int path0, bool last, bool seeding, uint8_t* lds, int lds_bytes) {
   const int tid = threadIdx.x;
   const bool keep_path = a.scores.result != 0;
   const bool with_len = a.scores.f > 0;
-  if (tid == 0) {
-    a.result[w].t_begin = now();
-    S.status = MPOA_OK;
-    S.n_nodes = 0; S.n_in_e = 0; S.n_out_e = 0;
-    S.cells = 0; S.edge_cells = 0;
-    for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
-    new_node(A, S, a.caps, CODE_N);  // source 0 (graph.cpp:129-135)
-    new_node(A, S, a.caps, CODE_N);  // sink 1
+  {
+    const int64_t ta = now();
+    if (!add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
+      if (tid == 0) add_path(A, S, a.caps, keep_path, r, m, path0);
+    }
+    __syncthreads();
+    if (tid == 0) S.ticks[PH_ADD] += now() - ta;
   }
+  if (S.status != MPOA_OK) return;
+  const int64_t tb = now();
+  if (last) {
+    if (tid == 0) S.msa_cols = topsort1(A, S);
+  } else {
+    topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes);
+    if (seeding && tid == 0) S.cons_len = build_consensus_path(A, S);
+  }
+  if (tid == 0) S.ticks[PH_SORT] += now() - tb;
   __syncthreads();
-  int path0 = 0;
-  for (int r = 0; r < wd.n_reads; r++) {
-    const int64_t q0 = a.read_off[wd.read0 + r], q1 = a.read_off[wd.read0 + r + 1];
-    const int qlen = (int)(q1 - q0);
-    const int m = qlen + 1;
-    if (qlen > a.caps.qmax || (keep_path && path0 + qlen > a.caps.path)) {
-      if (tid == 0) S.status = MPOA_E_OVERFLOW;
-      __syncthreads();
-      break;
-    }
-    // Query: sentinel N, codes, N padding to col_size (align.cpp:1362-1373).
-    const int col_size = (m + 1 + Q - 1) / Q * Q;
-    // qsh / R.q: the same codes shifted one column right (entry c is the character column c
-    // consumes in the M-source, seq[c-1]; entry 0 is N).
-    const bool q_lds = col_size + 2 * Q <= RingT::kQ;
-    for (int j = tid; j < col_size + 2 * Q; j += NT) {
-      const uint8_t cq =
-          j >= 1 && j <= qlen ? (uint8_t)code_of(a.read_seq[q0 + j - 1]) : (uint8_t)CODE_N;
-      const uint8_t cs =
-          j >= 2 && j <= qlen + 1 ? (uint8_t)code_of(a.read_seq[q0 + j - 2]) : (uint8_t)CODE_N;
-      A.qseq[j] = cq;
-      A.qsh[j] = cs;
-      if (q_lds) R.q[j] = cs;
-    }
-    if (tid == 0) {
-      S.read = r; S.qlen = qlen; S.m = m; S.q_lds = q_lds;
-      S.q_cap = col_size + 2 * Q;
-    }
-    __syncthreads();
-    if (S.n_nodes <= 2) {
-      // First read: source then insertions (align.cpp:1747-1753).
-      for (int j = tid; j < m; j += NT) {
-        A.res_from[j] = j == 0 ? 0 : -1;
-        A.res_base[j] = j == 0 ? (uint8_t)CODE_N : A.qseq[j];
-        A.res_aligned[j] = -1;
-      }
-    } else {
-      if (tid == 0) {
-        S.beg_i = A.rank[0];
-        S.end_i = A.rank[1];
-        S.n_rows = S.end_i - S.beg_i + 1;
-      }
-      __syncthreads();
-      const bool adaptive = a.scores.ab_band != 0 && a.scores.f > 0;
-      if (S.q_lds) poa_call<NT, CPL, true>(a, A, S, R, adaptive);
-      else poa_call<NT, CPL, false>(a, A, S, R, adaptive);
-      if (tid == 0 && a.call_stats && a.call_off[w] + r - 1 < a.call_off[w + 1]) {
-        CallStat& cs = a.call_stats[a.call_off[w] + r - 1];
-        cs.score = S.score;
-        cs.rows = S.n_rows;
-        cs.qlen = qlen;
-        cs.width = S.call_width;
-        cs.cells = S.call_cells;
-        cs.edge_cells = S.call_ecells;
-        cs.nodes = S.n_nodes;
-        cs.edges = S.n_in_e;
-      }
-    }
-    __syncthreads();
-    if (S.status != MPOA_OK) break;
-    {
-      const int64_t ta = now();
-      if (!add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
-        if (tid == 0) add_path(A, S, a.caps, keep_path, r, m, path0);
-      }
-      __syncthreads();
-      if (tid == 0) S.ticks[PH_ADD] += now() - ta;
-    }
-    if (S.status == MPOA_OK) {
-      const int64_t tb = now();
-      if (r + 1 == wd.n_reads) {
-        if (tid == 0) S.msa_cols = topsort1(A, S);
-      } else {
-        topsort0<NT>(A, S, a.caps, with_len, reinterpret_cast<uint8_t*>(R.H),
-                     (int)(sizeof(R.H) + sizeof(R.D)));
-      }
-      if (tid == 0) S.ticks[PH_SORT] += now() - tb;
-    }
-    path0 += qlen;
-    __syncthreads();
-    if (S.status != MPOA_OK) break;
-  }
-  // Outputs.
+}
+
+// Output text of a finished window and its result record.
+template <int NT>
+__device__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int w) {
+  const WinDesc& wd = a.win[w];
+  const int tid = threadIdx.x;
   const int64_t t_out = now();
   if (S.status == MPOA_OK) {
     if (a.scores.result == 0) {
@@ -1509,6 +1532,234 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R
   __syncthreads();
 }
 
+template <int NT>
+__device__ void window_begin(const KernelArgs& a, Arena& A, WinShared& S, int w) {
+  if (threadIdx.x == 0) {
+    a.result[w].t_begin = now();
+    S.status = MPOA_OK;
+    S.n_nodes = 0; S.n_in_e = 0; S.n_out_e = 0;
+    S.cells = 0; S.edge_cells = 0;
+    S.cons_len = 0;
+    for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
+    new_node(A, S, a.caps, CODE_N);  // source 0 (graph.cpp:129-135)
+    new_node(A, S, a.caps, CODE_N);  // sink 1
+  }
+  __syncthreads();
+}
+
+// The whole per-window pipeline (no seeding): every read against the whole graph.
+template <int NT, int CPL, class RingT>
+__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, int w) {
+  const WinDesc& wd = a.win[w];
+  const int tid = threadIdx.x;
+  const bool keep_path = a.scores.result != 0;
+  window_begin<NT>(a, A, S, w);
+  int path0 = 0;
+  for (int r = 0; r < wd.n_reads; r++) {
+    const int64_t q0 = a.read_off[wd.read0 + r], q1 = a.read_off[wd.read0 + r + 1];
+    const int qlen = (int)(q1 - q0);
+    const int m = qlen + 1;
+    if (qlen > a.caps.qmax || (keep_path && path0 + qlen > a.caps.path)) {
+      if (tid == 0) S.status = MPOA_E_OVERFLOW;
+      __syncthreads();
+      break;
+    }
+    load_query<NT>(A, S, R, a.read_seq + q0, qlen);
+    if (tid == 0) S.read = r;
+    if (S.n_nodes <= 2) {
+      trivial_res<NT>(A, m);
+    } else {
+      if (tid == 0) {
+        S.beg_i = A.rank[0];
+        S.end_i = A.rank[1];
+        S.n_rows = S.end_i - S.beg_i + 1;
+      }
+      __syncthreads();
+      const bool adaptive = a.scores.ab_band != 0 && a.scores.f > 0;
+      if (S.q_lds) poa_call<NT, CPL, true>(a, A, S, R, adaptive);
+      else poa_call<NT, CPL, false>(a, A, S, R, adaptive);
+      if (tid == 0 && a.call_stats && a.call_off[w] + r - 1 < a.call_off[w + 1]) {
+        CallStat& cs = a.call_stats[a.call_off[w] + r - 1];
+        cs.score = S.score;
+        cs.rows = S.n_rows;
+        cs.qlen = qlen;
+        cs.width = S.call_width;
+        cs.cells = S.call_cells;
+        cs.edge_cells = S.call_ecells;
+        cs.nodes = S.n_nodes;
+        cs.edges = S.n_in_e;
+      }
+    }
+    __syncthreads();
+    if (S.status != MPOA_OK) break;
+    commit_read<NT>(a, A, S, r, m, path0, r + 1 == wd.n_reads, false,
+                    reinterpret_cast<uint8_t*>(R.H), (int)(sizeof(R.H) + sizeof(R.D)));
+    path0 += qlen;
+    __syncthreads();
+    if (S.status != MPOA_OK) break;
+  }
+  window_outputs<NT>(a, A, S, w);
+}
+
+// ---------------------------------------------------------------------------------------
+// Seeding (-S) windows: alignment() splits each read at the anchors of its chain against the
+// consensus (align.cpp:1763-1827); the sub-windows are independent DP calls on the same
+// graph. The host seeds each read; k_segments runs every sub-window of the read in its own
+// workgroup, and k_commit splices the anchor bases, adds the path, re-sorts and builds the
+// consensus the next read is seeded against. The graph stays in slot 0 of the graph pools,
+// and the window state between launches in SeedArgs::state.
+// ---------------------------------------------------------------------------------------
+
+__device__ void state_load(WinShared& S, const WinState& st) {
+  S.n_nodes = st.n_nodes; S.n_in_e = st.n_in_e; S.n_out_e = st.n_out_e;
+  S.status = st.status; S.msa_cols = st.msa_cols; S.cons_len = st.cons_len;
+  S.cells = st.cells; S.edge_cells = st.edge_cells;
+  for (int k = 0; k < PH_N; k++) S.ticks[k] = st.ticks[k];
+}
+__device__ void state_store(const WinShared& S, WinState& st) {
+  st.n_nodes = S.n_nodes; st.n_in_e = S.n_in_e; st.n_out_e = S.n_out_e;
+  st.status = S.status; st.msa_cols = S.msa_cols; st.cons_len = S.cons_len;
+  st.cells = S.cells; st.edge_cells = S.edge_cells;
+  for (int k = 0; k < PH_N; k++) st.ticks[k] = S.ticks[k];
+}
+
+template <int NT, int CPL>
+__global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
+  __shared__ WinShared S;
+  __shared__ Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP> R;
+  using RingT = Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP>;
+  const int tid = threadIdx.x;
+  const SegDesc d = s.segs[blockIdx.x];
+  const Arena G = carve(s.graph, 0);
+  Arena A = carve(s.seg, blockIdx.x);
+  // The graph is read-only here; the DP plan, band state, query and matrices are private.
+  A.par = G.par; A.aligned = G.aligned; A.ind = G.ind; A.rank = G.rank;
+  A.in_head = G.in_head; A.in_tail = G.in_tail; A.out_head = G.out_head;
+  A.out_tail = G.out_tail; A.in_cnt = G.in_cnt; A.last_read = G.last_read;
+  A.out_cnt = G.out_cnt; A.ie_src = G.ie_src; A.ie_w = G.ie_w; A.ie_next = G.ie_next;
+  A.oe_dst = G.oe_dst; A.oe_w = G.oe_w; A.oe_next = G.oe_next; A.rank_id = G.rank_id;
+  A.hlen = G.hlen; A.tlen = G.tlen; A.pred_off = G.pred_off; A.pred_rank = G.pred_rank;
+  A.pred_w = G.pred_w; A.pred_base = G.pred_base; A.base = G.base;
+  A.res_from = G.res_from + d.res0;
+  A.res_aligned = G.res_aligned + d.res0;
+  A.res_base = G.res_base + d.res0;
+  SegResult& out = s.seg_res[blockIdx.x];
+  if (tid == 0) {
+    S.status = MPOA_OK;
+    S.n_nodes = s.state->n_nodes;
+    S.n_in_e = s.state->n_in_e;
+    S.cells = 0;
+    S.edge_cells = 0;
+    for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
+    S.beg_i = G.rank[d.beg_id];
+    S.end_i = G.rank[d.end_id];
+    S.n_rows = S.end_i - S.beg_i + 1;
+    S.read = s.read;
+  }
+  __syncthreads();
+  if (S.n_rows <= 0 || d.qlen + 1 > s.seg.caps.qmax + 1 || S.n_rows > s.seg.caps.nodes) {
+    if (tid == 0) {
+      out.status = S.n_rows <= 0 ? MPOA_E_EMPTY_WINDOW : MPOA_E_OVERFLOW;
+      out.score = 0; out.rows = S.n_rows; out.qlen = d.qlen;
+      out.cells = 0; out.edge_cells = 0; out.width = 0;
+    }
+    return;
+  }
+  // Row records of the window's ranks: the predecessor fields come from the graph; the
+  // DP plan overwrites its own fields in this private copy.
+  for (int e = tid; e < 12 * S.n_rows; e += NT) {
+    const int64_t k = 12LL * S.beg_i + e;
+    A.rowrec[k] = G.rowrec[k];
+  }
+  __syncthreads();
+  const WinDesc& wd = s.graph.win[0];
+  const int64_t q0 = s.graph.read_off[wd.read0 + s.read] + d.q0;
+  load_query<NT>(A, S, R, s.graph.read_seq + q0, d.qlen);
+  KernelArgs a = s.seg;
+  a.scores = s.graph.scores;
+  // -S always uses the adaptive band (align.cpp:1780, 1808, 1826).
+  const bool adaptive = a.scores.f > 0;
+  if (S.q_lds) poa_call<NT, CPL, true, RingT>(a, A, S, R, adaptive);
+  else poa_call<NT, CPL, false, RingT>(a, A, S, R, adaptive);
+  if (tid == 0) {
+    out.status = S.status;
+    out.score = S.score;
+    out.rows = S.n_rows;
+    out.qlen = d.qlen;
+    out.width = S.call_width;
+    out.cells = S.call_cells;
+    out.edge_cells = S.call_ecells;
+    for (int k = 0; k < PH_N; k++) out.ticks[k] = S.ticks[k];
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
+  __shared__ WinShared S;
+  __shared__ int4 lds4[(96 * 1024) / 16];
+  const int tid = threadIdx.x;
+  const KernelArgs& a = s.graph;
+  Arena A = carve(a, 0);
+  const WinDesc& wd = a.win[0];
+  const int r = s.read;
+  if (r == 0) {
+    window_begin<NT>(a, A, S, 0);
+  } else {
+    if (tid == 0) state_load(S, *s.state);
+    __syncthreads();
+  }
+  const int64_t q0 = a.read_off[wd.read0 + r], q1 = a.read_off[wd.read0 + r + 1];
+  const int qlen = (int)(q1 - q0), m = qlen + 1;
+  const bool keep_path = a.scores.result != 0;
+  if (S.status == MPOA_OK) {
+    if (qlen > a.caps.qmax || (keep_path && s.path0 + qlen > a.caps.path)) {
+      if (tid == 0) S.status = MPOA_E_OVERFLOW;
+    } else if (s.n_seg == 0) {
+      for (int j = tid; j < m; j += NT) {
+        A.res_from[j] = j == 0 ? 0 : -1;
+        A.res_base[j] = j == 0 ? (uint8_t)CODE_N : (uint8_t)code_of(a.read_seq[q0 + j - 1]);
+        A.res_aligned[j] = -1;
+      }
+    } else {
+      if (tid == 0) {
+        for (int k = 0; k < s.n_seg; k++) {
+          const SegResult& g = s.seg_res[k];
+          if (g.status != MPOA_OK && S.status == MPOA_OK) S.status = g.status;
+          S.cells += g.cells;
+          S.edge_cells += g.edge_cells;
+          for (int t = 0; t < PH_N; t++) S.ticks[t] += g.ticks[t];
+        }
+      }
+      // Anchor bases between sub-windows: existing consensus nodes (align.cpp:1794-1797).
+      for (int k = tid; k < s.n_splice; k += NT) {
+        const int pos = s.splice[2 * k], id = s.splice[2 * k + 1];
+        A.res_from[pos] = id;
+        A.res_base[pos] = A.base[id];
+        A.res_aligned[pos] = -1;
+      }
+    }
+  }
+  __syncthreads();
+  if (S.status == MPOA_OK) {
+    commit_read<NT>(a, A, S, r, m, s.path0, r + 1 == s.n_reads, true,
+                    reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
+  }
+  if (S.status == MPOA_OK && r + 1 < s.n_reads) {
+    if (S.cons_len > s.cons_cap) {
+      if (tid == 0) S.status = MPOA_E_OVERFLOW;
+    } else {
+      for (int p = tid; p < S.cons_len; p += NT) {
+        s.cons_out[3 * p] = A.cons_id[p];
+        s.cons_out[3 * p + 1] = A.cons_rank[p];
+        s.cons_out[3 * p + 2] = A.cons_ch[p];
+      }
+    }
+    __syncthreads();
+  }
+  if (r + 1 == s.n_reads || S.status != MPOA_OK) window_outputs<NT>(a, A, S, 0);
+  if (tid == 0) state_store(S, *s.state);
+}
+
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
   __shared__ WinShared S;
@@ -1548,4 +1799,23 @@ int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
   MPOA_TILES(MPOA_L)
 #undef MPOA_L
   return -2;
+}
+
+int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream) {
+  using namespace mpoa;
+  if (s.n_seg <= 0) return 0;
+#define MPOA_S(NT_, CPL_)                                                                     \
+  if (nt == NT_ && cpl == CPL_) {                                                            \
+    hipLaunchKernelGGL((k_segments<NT_, CPL_>), dim3(s.n_seg), dim3(NT_), 0, stream, s);     \
+    return hipGetLastError() == hipSuccess ? 0 : -1;                                         \
+  }
+  MPOA_S(64, 4) MPOA_S(128, 4) MPOA_S(256, 2)
+#undef MPOA_S
+  return -2;
+}
+
+int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream) {
+  using namespace mpoa;
+  hipLaunchKernelGGL((k_commit<256>), dim3(1), dim3(256), 0, stream, s);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -11,8 +11,6 @@ CASES = gu.load_cases()
 
 def _unsupported(case) -> str | None:
     p = gu.parse_args(case["args"])
-    if p.get("seeding"):
-        return "-S seeding is not in the device path yet"
     if p.get("result") == 2:
         return "GFA output is not in the device path yet"
     return None

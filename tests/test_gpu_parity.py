@@ -85,3 +85,14 @@ def test_batch_matches_oracle(ctx):
     got = bat.results().texts
     want = oracle_lib.batch(p, b.win_off, b.seq, b.seq_off)
     assert got == want
+
+
+@pytest.mark.parametrize("result", [0, 1])
+def test_seeded_windows(ctx, result):
+    """-S: host seeding, one workgroup per sub-window, adaptive band (align.cpp:1763-1827)."""
+    reads = synth.window(31, 1500, 14, 0.04, 0.03, 0.03, 0.7)
+    _say("seeded config2-like", result)
+    _check_window(ctx, reads, seeding=1, result=result)
+    reads = synth.window(32, 5000, 6, 0.024, 0.048, 0.048, 0.7)
+    _say("seeded config4-like", result)
+    _check_window(ctx, reads, seeding=1, k=15, mm_w=10, result=result)
