@@ -168,20 +168,22 @@ class Context:
         nb, no = _names(names)
         cb, co = _names(comments)
         need = ctypes.c_int64()
-        stats = (CallStat * max(n, 1))()
         n_stats = ctypes.c_int32()
         rc = lib().mpoa_align_reads(self.h, ctypes.byref(params), n, _ptr(seq), _ptr(off),
                                     _ptr(nb), _ptr(no), _ptr(cb), _ptr(co), None, 0,
-                                    ctypes.byref(need), stats, max(n, 1), ctypes.byref(n_stats))
+                                    ctypes.byref(need), None, 0, ctypes.byref(n_stats))
         _check(rc)
         buf = np.zeros(max(need.value, 1), np.uint8)
+        cap = max(n_stats.value, 1)
+        stats = (CallStat * cap)()
         rc = lib().mpoa_align_reads(self.h, ctypes.byref(params), n, _ptr(seq), _ptr(off),
                                     _ptr(nb), _ptr(no), _ptr(cb), _ptr(co), _ptr(buf),
-                                    len(buf), ctypes.byref(need), None, 0, None)
+                                    len(buf), ctypes.byref(need), stats, cap,
+                                    ctypes.byref(n_stats))
         _check(rc)
         text = buf[:need.value].tobytes().decode()
         if with_stats:
-            return text, [stats[i] for i in range(n_stats.value)]
+            return text, [stats[i] for i in range(min(n_stats.value, cap))]
         return text
 
 
