@@ -200,6 +200,7 @@ struct mpoa_batch {
   int launches = 0;
   int first_slots = 0;
   std::vector<mpoa_launch_stat> lstats;  // launches of the last execute
+  DevBuf<int64_t> d_gfa;                  // one-window runs: offsets of the GFA path lines
   int mode = MPOA_MODE_BATCHED;
   // Learned predictions per window (mpoa_batch_set_predictions): DP cells of the costliest
   // call, widest band row and final node count. Empty when none were given.
@@ -299,6 +300,7 @@ int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
   a.out = b->d_out.p;
   a.call_stats = b->d_calls.p;
   a.call_off = b->d_call_off.p;
+  a.gfa_lines = b->d_gfa.p;
   const int lrc = mpoa_launch_windows(a, bin.used_slots, bin.nt, bin.cpl, stream);
   if (lrc != 0) {
     g_err = lrc == -2 ? "unsupported tile shape" : "kernel launch failed";
@@ -586,6 +588,7 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
     ga.byte_pool = g_byte.p;
     ga.dp_pool = g_dp.p;
     ga.out = b->d_out.p;
+    ga.gfa_lines = b->d_gfa.p;
     SlotCaps sc_caps{};
     size_t seg_slots = 0;
     int64_t seg_dp = 0;
@@ -816,8 +819,8 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
     g_err = "seeding (-S) windows are not supported by the batch pipeline";
     return MPOA_E_ARG;
   }
-  if (params->result == 2) {
-    g_err = "GFA output is not supported by the batch pipeline";
+  if (params->result < 0 || params->result > 2) {
+    g_err = "result must be 0 (consensus), 1 (MSA) or 2 (GFA)";
     return MPOA_E_ARG;
   }
   HIPCHK(hipSetDevice(ctx->device));
@@ -846,8 +849,17 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
     d.n_reads = wi.n_reads;
     d.out0 = out0;
     int64_t cap;
-    if (params->result == 1) cap = (int64_t)wi.n_reads * (2 + wi.qtot);
-    else cap = 2 * (int64_t)wi.qmax + 64;
+    if (params->result == 1) {
+      cap = (int64_t)wi.n_reads * (2 + wi.qtot);
+    } else if (params->result == 2) {
+      // output_gfa bound: nodes <= 2 + sum q, in-edges <= sum q + reads, d-digit ids
+      int d = 1;
+      for (int64_t v = 2 + wi.qtot; v >= 10; v /= 10) d++;
+      cap = (2 + wi.qtot) * (6 + d) + (wi.qtot + wi.n_reads) * (13 + 2 * d) +
+            wi.qtot * (d + 2) + (int64_t)wi.n_reads * (16 + d) + 64;
+    } else {
+      cap = 2 * (int64_t)wi.qmax + 64;
+    }
     cap = std::min<int64_t>(cap, INT32_MAX / 2);
     d.out_cap = (int)cap;
     out0 += (cap + 15) / 16 * 16;
@@ -1027,6 +1039,8 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
       text.assign(kHead);
       text.append((const char*)src, (size_t)std::max(0, r.out_len));
       text.push_back('\n');
+    } else if (b->params.result == 2) {
+      text.assign((const char*)src, (size_t)std::max(0, r.out_len));
     } else {
       const int L = std::max(0, r.out_len);
       for (int i = 0; i < b->info[w].n_reads; i++) {
@@ -1182,10 +1196,6 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
                      int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
                      int32_t stats_cap, int32_t* n_stats) {
   if (!ctx || !params || n_reads < 0 || (n_reads > 0 && (!seq || !seq_off))) return MPOA_E_ARG;
-  if (params->result == 2) {
-    g_err = "GFA output (-r 2) is not supported";
-    return MPOA_E_ARG;
-  }
   int32_t win_off[2] = {0, n_reads};
   std::vector<int64_t> zero_off{0};
   const int64_t* so = n_reads > 0 ? seq_off : zero_off.data();
@@ -1198,6 +1208,10 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   int rc = mpoa_batch_prepare(ctx, &pp, 1, win_off, seq ? seq : "", so, nullptr, &b);
   if (rc != MPOA_OK) return rc;
   b->params.seeding = params->seeding;
+  if (params->result == 2 && (rc = b->d_gfa.alloc((size_t)n_reads + 1)) != MPOA_OK) {
+    mpoa_batch_free(b);
+    return rc;
+  }
   std::vector<mpoa_call_stat> calls;
   if (seeding) {
     rc = run_seeded(b, seq, so, &calls);
@@ -1235,6 +1249,31 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
     text = ">consensus sequence\n";
     text.append(cons.data(), (size_t)r.out_len);
     text.push_back('\n');
+  } else if (params->result == 2) {
+    // output_gfa (src/graph.cpp:358-433): the device names path lines by read index; the
+    // read names replace those where given.
+    std::vector<char> g((size_t)std::max(r.out_len, 1));
+    std::vector<int64_t> at((size_t)n_reads + 1);
+    if (r.out_len > 0 &&
+        (hipMemcpy(g.data(), b->d_out.p, (size_t)r.out_len, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(at.data(), b->d_gfa.p, at.size() * sizeof(int64_t), hipMemcpyDeviceToHost) !=
+             hipSuccess)) {
+      mpoa_batch_free(b);
+      return MPOA_E_DEVICE;
+    }
+    if (r.out_len > 0) {
+      text.assign(g.data(), (size_t)at[0]);
+      for (int i = 0; i < n_reads; i++) {
+        const char* line = g.data() + at[i];
+        const int64_t len = at[i + 1] - at[i];
+        const char* tab = (const char*)std::memchr(line + 2, '\t', (size_t)(len - 2));
+        std::string nm;
+        if (name && name_off) nm.assign(name + name_off[i], name + name_off[i + 1]);
+        text += "P\t";
+        text += nm.empty() ? std::to_string(i + 1) : nm;
+        if (tab) text.append(tab, (size_t)(line + len - tab));
+      }
+    }
   } else {
     // output_rc_msa (src/graph.cpp:237-251)
     const int L = r.out_len;

@@ -76,6 +76,7 @@ struct KernelArgs {
   uint8_t* byte_pool;
   int* dp_pool;
   uint8_t* out;
+  int64_t* gfa_lines;       // optional (one window): offset of each read's GFA path line
   CallStat* call_stats;     // optional: per DP call, window w's calls from call_off[w]
   const int64_t* call_off;  // n_windows + 1 offsets (reads - 1 per window)
 };

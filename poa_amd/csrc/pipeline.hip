@@ -1455,6 +1455,12 @@ __device__ void commit_read(const KernelArgs& a, Arena& A, WinShared& S, int r, 
   const int tid = threadIdx.x;
   const bool keep_path = a.scores.result != 0;
   const bool with_len = a.scores.f > 0;
+  if (keep_path) {
+    // positions add_path does not record (a repeated node) print as node 1 - 1 = 0, as
+    // output_gfa's zero-initialised path does (graph.cpp:365)
+    for (int j = tid; j < m - 1; j += NT) A.path[path0 + j] = 1;
+    __syncthreads();
+  }
   {
     const int64_t ta = now();
     if (!add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
@@ -1473,6 +1479,82 @@ __device__ void commit_read(const KernelArgs& a, Arena& A, WinShared& S, int r, 
   }
   if (tid == 0) S.ticks[PH_SORT] += now() - tb;
   __syncthreads();
+}
+
+__device__ __forceinline__ void put_str(uint8_t* o, int64_t& at, const char* t) {
+  while (*t) o[at++] = (uint8_t)*t++;
+}
+__device__ __forceinline__ void put_uint(uint8_t* o, int64_t& at, unsigned v) {
+  char d[10];
+  int n = 0;
+  do {
+    d[n++] = (char)('0' + v % 10);
+    v /= 10;
+  } while (v);
+  while (n) o[at++] = (uint8_t)d[--n];
+}
+
+// graph::output_gfa (graph.cpp:358-433), one lane: segments and links in FIFO BFS order from
+// the source (a node's links in its in-list order, the source's left out), then one path line
+// per read named by its 1-based index. Segment ids are node ids - 1. Returns the text length,
+// or -1 when the capacity is short. line_at (optional) receives each path line's offset.
+__device__ int64_t gfa_text(const KernelArgs& a, Arena& A, WinShared& S, const WinDesc& wd) {
+  uint8_t* o = a.out + wd.out0;
+  const int64_t cap = wd.out_cap;
+  const int N = S.n_nodes;
+  if (N <= 2) return 0;
+  int64_t at = 0;
+  put_str(o, at, "H\tVN:Z:1.0\n");
+  int* deg = A.t0;
+  int* queue = A.t1;
+  for (int v = 0; v < N; v++) deg[v] = A.in_cnt[v];
+  int qh = 0, qt = 0;
+  queue[qt++] = 0;
+  while (qh < qt) {
+    const int cur = queue[qh++];
+    if (cur == 1) break;
+    if (cur != 0) {
+      if (at + 32 > cap) return -1;
+      put_str(o, at, "S\t");
+      put_uint(o, at, (unsigned)(cur - 1));
+      o[at++] = '\t';
+      o[at++] = char_of(A.base[cur]);
+      o[at++] = '\n';
+      for (int e = A.in_head[cur]; e != -1; e = A.ie_next[e]) {
+        const int pre = A.ie_src[e];
+        if (pre == 0) continue;
+        if (at + 48 > cap) return -1;
+        put_str(o, at, "L\t");
+        put_uint(o, at, (unsigned)(pre - 1));
+        put_str(o, at, "\t+\t");
+        put_uint(o, at, (unsigned)(cur - 1));
+        put_str(o, at, "\t+\t0M\n");
+      }
+    }
+    for (int e = A.out_head[cur]; e != -1; e = A.oe_next[e]) {
+      const int t = A.oe_dst[e];
+      if (--deg[t] == 0) queue[qt++] = t;
+    }
+  }
+  int64_t p0 = 0;
+  for (int r = 0; r < wd.n_reads; r++) {
+    const int qlen = (int)(a.read_off[wd.read0 + r + 1] - a.read_off[wd.read0 + r]);
+    if (a.gfa_lines) a.gfa_lines[r] = at;
+    if (at + 32 > cap) return -1;
+    put_str(o, at, "P\t");
+    put_uint(o, at, (unsigned)(r + 1));
+    o[at++] = '\t';
+    for (int j = 0; j < qlen; j++) {
+      if (at + 16 > cap) return -1;
+      put_uint(o, at, (unsigned)(A.path[p0 + j] - 1));
+      o[at++] = '+';
+      if (j != qlen - 1) o[at++] = ',';
+      else put_str(o, at, "\t*\n");
+    }
+    p0 += qlen;
+  }
+  if (a.gfa_lines) a.gfa_lines[wd.n_reads] = at;
+  return at;
 }
 
 // Output text of a finished window and its result record.
@@ -1513,6 +1595,13 @@ __device__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int 
           p0 += qlen;
         }
         if (tid == 0) S.cons_len = L;
+      }
+      __syncthreads();
+    } else if (a.scores.result == 2) {
+      if (tid == 0) {
+        const int64_t len = gfa_text(a, A, S, wd);
+        if (len < 0 || len > INT32_MAX) S.status = MPOA_E_OVERFLOW;
+        else S.cons_len = (int)len;
       }
       __syncthreads();
     }

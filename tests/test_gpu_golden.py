@@ -9,19 +9,9 @@ import golden_util as gu
 CASES = gu.load_cases()
 
 
-def _unsupported(case) -> str | None:
-    p = gu.parse_args(case["args"])
-    if p.get("result") == 2:
-        return "GFA output is not in the device path yet"
-    return None
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", CASES, ids=[gu.case_id(c) for c in CASES])
 def test_device_matches_upstream(ctx, case):
-    why = _unsupported(case)
-    if why:
-        pytest.skip(why)
     from poa_amd import api
     seq, off, names, comments = gu.parse_fasta(case["fasta"])
     p = api.Params(**gu.parse_args(case["args"]))
