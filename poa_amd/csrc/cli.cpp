@@ -1,0 +1,222 @@
+// minipoa_amd: minipoa's command line (src/main.cpp:14-35) over the device library.
+//
+//   minipoa_amd [options] reads.fa[.gz]            one read set, minipoa's stdout
+//   minipoa_amd [options] w1.fa w2.fa ...           many read sets as one batch (no -S):
+//                                                   their outputs, in argument order
+//
+// Options as minipoa: -M -X -O -E scores, -b -f band, -B adaptive band, -S seeding with -k -w
+// -W, -r 0|1|2 consensus / row-column MSA / GFA, -V 1 prints "score:<n>" per DP call to stderr
+// (src/align.cpp:1577), -t is accepted and ignored (one device). Exit status follows minipoa:
+// 0 after a band violation (align.cpp:1584-1588), 1 after a backtrack failure (:1730-1731).
+#include <zlib.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpoa.h"
+
+namespace {
+
+struct ReadSet {
+  std::string seq, name, comment;
+  std::vector<int64_t> seq_off{0}, name_off{0}, comment_off{0};
+  int n() const { return (int)seq_off.size() - 1; }
+};
+
+// FASTA / FASTQ (optionally gzip) in kseq's split: name = header up to the first blank,
+// comment = the rest; sequence lines joined (src/handle_input.cpp:13-43).
+bool load(const char* path, ReadSet& rs) {
+  gzFile fp = gzopen(path, "r");
+  if (!fp) return false;
+  std::string all;
+  char buf[1 << 16];
+  int got;
+  while ((got = gzread(fp, buf, sizeof buf)) > 0) all.append(buf, (size_t)got);
+  gzclose(fp);
+  size_t at = 0;
+  auto next_line = [&](std::string& line) {
+    if (at >= all.size()) return false;
+    size_t e = all.find('\n', at);
+    if (e == std::string::npos) e = all.size();
+    line.assign(all, at, e - at);
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    at = e + 1;
+    return true;
+  };
+  std::string line;
+  bool have = false;
+  std::string head, body;
+  bool fastq = false;
+  auto flush = [&]() {
+    if (!have) return;
+    const size_t sp = head.find_first_of(" \t");
+    rs.name += sp == std::string::npos ? head : head.substr(0, sp);
+    rs.comment += sp == std::string::npos ? std::string() : head.substr(sp + 1);
+    rs.seq += body;
+    rs.seq_off.push_back((int64_t)rs.seq.size());
+    rs.name_off.push_back((int64_t)rs.name.size());
+    rs.comment_off.push_back((int64_t)rs.comment.size());
+  };
+  while (next_line(line)) {
+    if (line.empty()) continue;
+    if (line[0] == '>' || line[0] == '@') {
+      flush();
+      have = true;
+      fastq = line[0] == '@';
+      head = line.substr(1);
+      body.clear();
+      continue;
+    }
+    if (!have) continue;
+    if (fastq && line[0] == '+') {
+      // skip as many quality characters as there are bases
+      size_t q = 0;
+      while (q < body.size() && next_line(line)) q += line.size();
+      continue;
+    }
+    for (char c : line)
+      if (c != ' ' && c != '\t') body.push_back(c);
+  }
+  flush();
+  return true;
+}
+
+void usage() {
+  std::fprintf(stderr,
+               "usage: minipoa_amd [options] reads.fa [more.fa ...]\n"
+               "  -M INT match (2)      -X INT mismatch (-4)  -O INT gap open (-4)\n"
+               "  -E INT gap extend (-2) -b INT band b (100)  -f INT band f (40)\n"
+               "  -B adaptive band      -S seeding           -k INT k-mer (19)\n"
+               "  -w INT window (10)    -W INT sub-window     -r INT 0 consensus, 1 MSA, 2 GFA\n"
+               "  -V INT verbosity      -t INT threads (ignored)\n");
+}
+
+int exit_status(int rc) {
+  if (rc == MPOA_OK) return 0;
+  if (rc == MPOA_E_BAND) return 0;
+  return 1;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  mpoa_params p = {2, -4, -4, -2, 100, 40, 0, 0, 19, 10, 0, 0};
+  int verbose = 0;
+  std::vector<const char*> files;
+  for (int i = 1; i < argc; i++) {
+    const char* s = argv[i];
+    if (s[0] != '-' || s[1] == '\0') {
+      files.push_back(s);
+      continue;
+    }
+    const char opt = s[1];
+    auto value = [&]() -> int {
+      if (s[2]) return std::atoi(s + 2);
+      if (i + 1 >= argc) {
+        std::fprintf(stderr, "option -%c needs a value\n", opt);
+        std::exit(1);
+      }
+      return std::atoi(argv[++i]);
+    };
+    switch (opt) {
+      case 'M': p.match = value(); break;
+      case 'X': p.mismatch = value(); break;
+      case 'O': p.gap_open = value(); break;
+      case 'E': p.gap_ext = value(); break;
+      case 'b': p.band_b = value(); break;
+      case 'f': p.band_f = value(); break;
+      case 'B': p.ab_band = 1; break;
+      case 'S': p.seeding = 1; break;
+      case 'k': p.k = value(); break;
+      case 'w': p.mm_w = value(); break;
+      case 'W': p.poa_w = value(); break;
+      case 'r': p.result = value(); break;
+      case 'V': verbose = value(); break;
+      case 't': (void)value(); break;
+      case 'h': usage(); return 0;
+      default:
+        std::fprintf(stderr, "unknown option -%c\n", opt);
+        usage();
+        return 1;
+    }
+  }
+  if (files.empty()) {
+    usage();
+    return 1;
+  }
+  std::vector<ReadSet> sets(files.size());
+  for (size_t f = 0; f < files.size(); f++) {
+    if (!load(files[f], sets[f])) {
+      std::fprintf(stderr, "error read file: cannot open %s\n", files[f]);
+      return 1;
+    }
+  }
+  mpoa_ctx* ctx = nullptr;
+  int rc = mpoa_create(0, &ctx);
+  if (rc != MPOA_OK) {
+    std::fprintf(stderr, "minipoa_amd: %s\n", mpoa_strerror(rc));
+    return 1;
+  }
+  if (sets.size() == 1 || p.seeding) {
+    // minipoa's own run, one read set at a time
+    for (const ReadSet& rs : sets) {
+      int64_t need = 0;
+      int32_t n_calls = 0;
+      rc = mpoa_align_reads(ctx, &p, rs.n(), rs.seq.data(), rs.seq_off.data(), rs.name.data(),
+                            rs.name_off.data(), rs.comment.data(), rs.comment_off.data(), nullptr,
+                            0, &need, nullptr, 0, &n_calls);
+      std::vector<mpoa_call_stat> calls((size_t)std::max(n_calls, 1));
+      std::string out((size_t)std::max<int64_t>(need, 1), '\0');
+      if (rc == MPOA_OK)
+        rc = mpoa_align_reads(ctx, &p, rs.n(), rs.seq.data(), rs.seq_off.data(),
+                              rs.name.data(), rs.name_off.data(), rs.comment.data(),
+                              rs.comment_off.data(), &out[0], need, &need, calls.data(),
+                              (int32_t)calls.size(), &n_calls);
+      if (verbose)
+        for (int c = 0; c < n_calls && c < (int)calls.size(); c++)
+          std::fprintf(stderr, "score:%d\n", calls[c].score);
+      if (rc != MPOA_OK) {
+        std::fprintf(stderr, "minipoa_amd: %s\n", mpoa_strerror(rc));
+        mpoa_destroy(ctx);
+        return exit_status(rc);
+      }
+      std::fwrite(out.data(), 1, (size_t)need, stdout);
+    }
+    mpoa_destroy(ctx);
+    return 0;
+  }
+  // Many read sets: one batch, one device pass.
+  std::string seq;
+  std::vector<int64_t> seq_off{0};
+  std::vector<int32_t> win_off{0};
+  for (const ReadSet& rs : sets) {
+    for (int r = 0; r < rs.n(); r++) {
+      seq.append(rs.seq, (size_t)rs.seq_off[r], (size_t)(rs.seq_off[r + 1] - rs.seq_off[r]));
+      seq_off.push_back((int64_t)seq.size());
+    }
+    win_off.push_back(win_off.back() + rs.n());
+  }
+  mpoa_batch* b = nullptr;
+  rc = mpoa_batch_prepare(ctx, &p, (int32_t)sets.size(), win_off.data(), seq.data(),
+                          seq_off.data(), nullptr, &b);
+  if (rc == MPOA_OK) rc = mpoa_batch_execute(b);
+  std::vector<int64_t> off(sets.size() + 1);
+  mpoa_batch_stats st;
+  std::string text;
+  if (rc == MPOA_OK) rc = mpoa_batch_results(b, nullptr, 0, off.data(), &st);
+  if (rc == MPOA_OK) {
+    text.resize((size_t)std::max<int64_t>(off.back(), 1));
+    rc = mpoa_batch_results(b, &text[0], off.back(), off.data(), &st);
+  }
+  if (rc != MPOA_OK) {
+    std::fprintf(stderr, "minipoa_amd: %s\n", mpoa_strerror(rc));
+  } else {
+    std::fwrite(text.data(), 1, (size_t)off.back(), stdout);
+  }
+  if (b) mpoa_batch_free(b);
+  mpoa_destroy(ctx);
+  return exit_status(rc);
+}
