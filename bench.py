@@ -127,6 +127,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", default=None,
+                    help="small_qmax,small_width,move_top for the launch split")
     ap.add_argument("--no-learned", action="store_true",
                     help="schedule by the read-length estimate instead of the trained network")
     ap.add_argument("--plain-sample", type=int, default=100,
@@ -176,6 +178,8 @@ def main() -> int:
         b.set_predictions(pred)
         learned = {"model": os.path.relpath(sched.MODEL, ROOT), "predict_s":
                    round(time.perf_counter() - tl, 3), "trained_on": model.meta}
+    if args.schedule:
+        b.set_schedule(*[int(x) for x in args.schedule.split(",")])
     prepare_s = time.perf_counter() - tp
     if args.nt or args.cpl:
         b.set_tile(args.nt or 64, args.cpl or 4)
@@ -201,8 +205,8 @@ def main() -> int:
     results_s = time.perf_counter() - tr
     st = res.stats
     ws, tick_s = b.window_stats()
-    ph = np.array([[w.ticks[k] for k in range(6)] for w in ws], dtype=np.float64) * tick_s
-    slow = int(np.argmax(ph.sum(axis=1)))
+    ph = np.array([[w.ticks[k] for k in range(8)] for w in ws], dtype=np.float64) * tick_s
+    slow = int(np.argmax(ph[:, :6].sum(axis=1)))
     log("phase seconds summed over windows: " +
         ", ".join(f"{n}={v:.3f}" for n, v in zip(api.PHASES, ph.sum(axis=0))))
     log(f"slowest window {slow}: nodes={ws[slow].nodes} cells={ws[slow].cells} " +

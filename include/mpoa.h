@@ -113,6 +113,11 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
  * or 512) and `cols` DP columns per thread per pass (1, 2, 4 or 8). Not every pair is built;
  * returns MPOA_E_ARG for one that is not. Chosen by the autotuner (poa_amd/autotune.py). */
 int mpoa_batch_set_tile(mpoa_batch* b, int32_t threads, int32_t cols);
+/* Launch split: windows whose longest read is at most small_qmax and whose band is at most
+ * small_width columns run on the small tile; of the others' launch, move_top (-1: half a
+ * window per CU) of the costliest small-tile windows join it. Defaults 1500, 360, -1. */
+int mpoa_batch_set_schedule(mpoa_batch* b, int32_t small_qmax, int32_t small_width,
+                            int32_t move_top);
 /* Runs the whole per-window pipeline on the device for every window; inputs are already
  * resident. Synchronous. */
 int mpoa_batch_execute(mpoa_batch* b);
@@ -123,12 +128,13 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
 void mpoa_batch_free(mpoa_batch* b);
 
 /* Per-window diagnostics of the last execute: final graph size, DP cells and device time per
- * pipeline phase (DP plan, DP rows, backtrack, add_path, topological sort, output), in ticks
+ * pipeline phase (DP plan, DP rows, backtrack, add_path, topological sort, output; then the
+ * sort's two parts: the order itself and hlen/tlen), in ticks
  * of the device wall clock (*tick_hz ticks per second). */
 typedef struct mpoa_window_stat {
   int32_t status, nodes, edges, launch;  /* launch: 0 small-tile, 1 wide-tile, 2 re-run */
   int64_t cells, edge_cells;
-  int64_t ticks[6];
+  int64_t ticks[8];
   int64_t t_begin, t_end;                /* device wall clock at the window's start and end */
 } mpoa_window_stat;
 int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);

@@ -28,14 +28,15 @@ EXPORTS = [
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
     "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_predictions",
-    "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
+    "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
     "mpoa_mlp_train", "mpoa_mlp_error",
 ]
 
 MODE_BATCHED, MODE_PLAIN = 0, 1
 
-PHASES = ("plan", "dp", "backtrack", "add_path", "topsort", "output")
+PHASES = ("plan", "dp", "backtrack", "add_path", "topsort", "output", "topsort.order",
+          "topsort.depth")
 
 
 class MpoaError(RuntimeError):
@@ -69,7 +70,7 @@ class CallStat(ctypes.Structure):
 class WindowStat(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32),
                 ("launch", ctypes.c_int32), ("cells", ctypes.c_int64),
-                ("edge_cells", ctypes.c_int64), ("ticks", ctypes.c_int64 * 6),
+                ("edge_cells", ctypes.c_int64), ("ticks", ctypes.c_int64 * 8),
                 ("t_begin", ctypes.c_int64), ("t_end", ctypes.c_int64)]
 
 
@@ -114,6 +115,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_launch_stats.argtypes = [VP, VP, I32, P(I32)]
         L.mpoa_batch_set_traces.argtypes = [VP, I32]
         L.mpoa_batch_set_mode.argtypes = [VP, I32]
+        L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
         _lib = L
@@ -225,6 +227,11 @@ class Batch:
         hz = ctypes.c_double()
         _check(lib().mpoa_batch_window_stats(self.h, arr, self.n, ctypes.byref(hz)))
         return [arr[i] for i in range(self.n)], 1.0 / hz.value
+
+    def set_schedule(self, small_qmax: int = 1500, small_width: int = 360,
+                     move_top: int = -1) -> None:
+        """Launch split between the small tile and the large launch."""
+        _check(lib().mpoa_batch_set_schedule(self.h, small_qmax, small_width, move_top))
 
     def set_mode(self, mode: int) -> None:
         """MODE_BATCHED (persistent launches, work queue) or MODE_PLAIN (one launch per

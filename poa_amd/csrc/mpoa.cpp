@@ -193,6 +193,9 @@ struct mpoa_batch {
   Bin small, large, rerun;
   std::vector<int> bin_of;    // launch of each window (0 small, 1 wide, 2 re-run)
   bool split = true;          // large windows in their own concurrent launch
+  // schedule knobs (mpoa_batch_set_schedule): windows the small tile takes, and how many of
+  // the costliest others join the large launch (-1: half a window per CU)
+  int small_qmax = 1500, small_width = 360, move_top = -1;
   std::vector<WinResult> res;
   int64_t reruns = 0;
   double device_ms = 0;
@@ -396,7 +399,7 @@ void assign_bins(mpoa_batch* b) {
   for (int w : b->order) {
     const WinInfo& wi = b->info[w];
     const int width = b->pred_width.empty() ? width_of(wi) : (int)b->pred_width[w];
-    const bool large = !b->split || wi.qmax > 1500 || width > 360;
+    const bool large = !b->split || wi.qmax > b->small_qmax || width > b->small_width;
     if (large && (wi.qmax > 4000 || width > 496)) need_wide = true;
     (large ? b->large.wins : b->small.wins).push_back(w);
   }
@@ -405,7 +408,7 @@ void assign_bins(mpoa_batch* b) {
   b->bin_of.assign(b->n_windows, 0);
   // The costliest remaining windows join the other launch up to half a window per CU: they
   // set the makespan, and the larger tile shortens their DP rows and sorts their graphs in LDS.
-  const size_t target = (size_t)std::max(1, b->ctx->n_cu / 2);
+  const size_t target = (size_t)std::max(0, b->move_top >= 0 ? b->move_top : b->ctx->n_cu / 2);
   if (b->split && b->large.wins.size() < target && !b->small.wins.empty()) {
     const size_t take = std::min(target - b->large.wins.size(), b->small.wins.size() / 8);
     std::vector<int> moved(b->small.wins.begin(), b->small.wins.begin() + take);
@@ -1079,7 +1082,7 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
     o.t_end = r.t_end;
     o.cells = r.cells;
     o.edge_cells = r.edge_cells;
-    for (int k = 0; k < 6; k++) o.ticks[k] = r.ticks[k];
+    for (int k = 0; k < 8; k++) o.ticks[k] = r.ticks[k];
   }
   return MPOA_OK;
 }
@@ -1180,6 +1183,16 @@ int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, i
   const int k = (int)b->lstats.size();
   for (int i = 0; i < k && i < cap; i++) out[i] = b->lstats[i];
   if (n) *n = k;
+  return MPOA_OK;
+}
+
+int mpoa_batch_set_schedule(mpoa_batch* b, int32_t small_qmax, int32_t small_width,
+                            int32_t move_top) {
+  if (!b || small_qmax < 0 || small_width < 0) return MPOA_E_ARG;
+  b->small_qmax = small_qmax;
+  b->small_width = small_width;
+  b->move_top = move_top;
+  assign_bins(b);
   return MPOA_OK;
 }
 

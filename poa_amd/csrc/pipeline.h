@@ -44,7 +44,8 @@ struct WinDesc {
 };
 
 // Device timers (wall_clock64 ticks) per phase of the window pipeline.
-enum Phase { PH_PLAN = 0, PH_DP, PH_BT, PH_ADD, PH_SORT, PH_OUT, PH_N };
+// PH_KAHN and PH_DEPTH are parts of PH_SORT (the order itself; hlen/tlen).
+enum Phase { PH_PLAN = 0, PH_DP, PH_BT, PH_ADD, PH_SORT, PH_OUT, PH_KAHN, PH_DEPTH, PH_N };
 
 struct WinResult {
   int status;

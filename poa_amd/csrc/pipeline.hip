@@ -116,6 +116,8 @@ struct WinShared {
   int64_t cells, edge_cells;
   int64_t call_cells, call_ecells;  // of the last DP call
   int call_width;                   // widest row of the last DP call
+  int bt_rc;                        // backtrack result of the last DP call
+  bool bt_serial;                   // it ran on one lane
   int64_t total;  // DP storage of the current call
   int job;
   int scan[16];
@@ -493,10 +495,12 @@ __device__ void tree_depth(int n, const int* par, int* depth, int* jmp, int base
 template <int NT>
 __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
                          int lds_bytes) {
+  const int64_t tk = now();
   if (!kahn_lds<NT>(A, S, lds, lds_bytes)) {
     if (threadIdx.x == 0) kahn_global(A, S);
     __syncthreads();
   }
+  if (threadIdx.x == 0) S.ticks[PH_KAHN] += now() - tk;
   const int n = S.n_rows;
   // DP plan in rank order: predecessor CSR offsets (block scan of in-degrees), then each lane
   // walks its rank's in-list.
@@ -552,8 +556,10 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
   }
   __syncthreads();
   if (with_len) {
+    const int64_t td = now();
     tree_depth<NT>(n, hpar, A.hlen, A.t2, 0);
     tree_depth<NT>(n, tchd, A.tlen, A.t2, 1);
+    if (threadIdx.x == 0) S.ticks[PH_DEPTH] += now() - td;
   }
 }
 
@@ -1145,6 +1151,189 @@ __device__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc) {
   return MPOA_OK;
 }
 
+// The backtrack (align.cpp:1572-1736, the same moves and tie-breaks as backtrack() above) by
+// one wavefront. Lane k < np holds the current row's k-th predecessor (in-list order) with
+// its band, row offset and row record; lane 63 reads the current cell. Each move costs one
+// round trip to HBM for the cells, plus one for the predecessors' records when the row
+// changes; the "largest in-weight, first wins" choices are wave maxima of
+// (weight << 6 | 63 - k). Moves are logged as local predecessor ranks (-1: insertion) in
+// res_from and turned into res_t entries afterwards by every thread (resolve_res).
+// Returns MPOA_OK, an error, or 1 when a row has more than 63 predecessors (the caller then
+// runs backtrack()). Called by wave 0 only.
+// Best candidate lane for the backtrack: largest weight, lowest lane; -1 if none.
+__device__ __forceinline__ int bt_pick(bool cand, int w, int lane) {
+  const int key = cand ? (w << 6) | (63 - lane) : -1;
+  const int best = __builtin_amdgcn_readlane(wave_incl_max(key), 63);
+  return best < 0 ? -1 : 63 - (best & 63);
+}
+
+__device__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc) {
+  const int lane = threadIdx.x & 63;
+  const int n = S.n_rows, m = S.m, beg = S.beg_i;
+  const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
+  int i = n - 1, acj = m, op = ALL_OP;
+  // current row (uniform)
+  int bs_i = A.Bs[i], be_i = A.Be[i], off_i = (int)A.row_off[i];
+  RowRec rr = load_rowrec(A.rowrec, beg + i);
+  // this lane's predecessor of the current row
+  int p = -1, pw = 0, pbase = 0, prule = 0, pbs = 0, pbe = 0, poff = 0;
+  RowRec prr{};
+  bool setup = true;
+  for (;;) {
+    if (setup) {
+      if (rr.np > 63) return 1;
+      p = -1;
+      if (lane < rr.np) {
+        int prank, b8;
+        if (lane < 3) {
+          prank = lane == 0 ? rr.p0 : lane == 1 ? rr.p1 : rr.p2;
+          pw = lane == 0 ? rr.w0 : lane == 1 ? rr.w1 : rr.w2;
+          b8 = ((rr.pk >> (3 * lane)) & 7) | (((rr.pk >> (9 + lane)) & 1) << 7);
+        } else {
+          prank = A.pred_rank[rr.k0 + lane];
+          pw = A.pred_w[rr.k0 + lane];
+          b8 = A.pred_base[rr.k0 + lane];
+        }
+        p = prank - beg;
+        pbase = p == 0 ? CODE_N : (b8 & 7);
+        prule = b8 >> 7;
+        if (p >= 0) {
+          pbs = A.Bs[p];
+          pbe = A.Be[p];
+          poff = (int)A.row_off[p];
+          prr = load_rowrec(A.rowrec, beg + p);
+        }
+      }
+      setup = false;
+    }
+    if (i <= 0 && acj <= 0) break;
+    const int j = acj - bs_i * Q;
+    if (acj > (be_i - 1) * Q || acj < bs_i * Q) return MPOA_E_BAND;
+    // One round trip: the predecessors' cells (lanes < np) and the current cell (lane 63).
+    const bool valid = p >= 0;
+    const int pj = acj - pbs * Q;
+    const bool m_in = valid && acj > 0 && pj - 1 >= 0 && pj - 1 < (pbe - pbs - 1) * Q;
+    const bool d_in = valid && pbs <= acj / Q && acj / Q < pbe - 1;
+    int hdiag = 0, hup = 0, dup = 0;
+    if (m_in) hdiag = A.H[poff + pj - 1];
+    if (d_in) {
+      hup = A.H[poff + pj];
+      dup = A.D[poff + pj];
+    }
+    int c_h = 0, c_d = 0, c_i = 0, c_hm = 0, c_im = 0, c_q = 0;
+    if (lane == 63) {
+      c_h = A.H[off_i + j];
+      c_d = A.D[off_i + j];
+      c_i = A.I[off_i + j];
+      if (j >= 1) {
+        c_hm = A.H[off_i + j - 1];
+        c_im = A.I[off_i + j - 1];
+      }
+      if (acj > 0) c_q = A.qseq[acj - 1];
+    }
+    const int h = __builtin_amdgcn_readlane(c_h, 63);
+    const int dv = __builtin_amdgcn_readlane(c_d, 63);
+    const int iv = __builtin_amdgcn_readlane(c_i, 63);
+    const int hm = __builtin_amdgcn_readlane(c_hm, 63);
+    const int im = __builtin_amdgcn_readlane(c_im, 63);
+    const int qc = __builtin_amdgcn_readlane(c_q, 63);
+    // kind: 1 substitution move (logged), 2 deletion move, 3 insertion (logged)
+    int k = -1, kind = 0, nop = op;
+    // 1. match-only move with the in-weight rule (align.cpp:1591-1626)
+    if ((op & M_OP) && acj > 0) {
+      k = bt_pick(m_in && pbase == qc && h == hdiag + match, pw, lane);
+      if (k >= 0 && __builtin_amdgcn_readlane(prule, k)) {
+        kind = 1;
+        nop = ALL_OP;
+      }
+    }
+    // 2. deletion: open first, then extend (align.cpp:1627-1667)
+    if (kind == 0 && (op & D_OP) && (op == D_OP || h == dv)) {
+      k = bt_pick(d_in && dv == hup + o1, pw, lane);
+      nop = M_OP | I_OP;
+      if (k < 0) {
+        k = bt_pick(d_in && dv == dup + e1, pw, lane);
+        nop = D_OP;
+      }
+      if (k >= 0) kind = 2;
+    }
+    // 3. insertion: open first, then extend (align.cpp:1668-1688)
+    if (kind == 0 && (op & I_OP) && acj > 0 && j - 1 >= 0 && (op == I_OP || h == iv)) {
+      if (iv == hm + o1) {
+        kind = 3;
+        nop = M_OP | D_OP;
+      } else if (iv == im + e1) {
+        kind = 3;
+        nop = I_OP;
+      }
+    }
+    // 4. any substitution, no in-weight rule (align.cpp:1689-1728)
+    if (kind == 0 && (op & M_OP) && acj > 0) {
+      k = bt_pick(m_in && h == hdiag + (pbase == qc ? match : mismatch), pw, lane);
+      if (k >= 0) {
+        kind = 1;
+        nop = ALL_OP;
+      }
+    }
+    if (kind == 0) return MPOA_E_BACKTRACK;
+    if (kind != 2) {
+      if (lane == 0) A.res_from[acj - 1] = kind == 1 ? __builtin_amdgcn_readlane(p, k) : -1;
+      acj--;
+    }
+    op = nop;
+    if (kind != 3) {
+      i = __builtin_amdgcn_readlane(p, k);
+      bs_i = __builtin_amdgcn_readlane(pbs, k);
+      be_i = __builtin_amdgcn_readlane(pbe, k);
+      off_i = __builtin_amdgcn_readlane(poff, k);
+      rr.row_off = __builtin_amdgcn_readlane(prr.row_off, k);
+      rr.bsbe = __builtin_amdgcn_readlane(prr.bsbe, k);
+      rr.k0 = __builtin_amdgcn_readlane(prr.k0, k);
+      rr.np = __builtin_amdgcn_readlane(prr.np, k);
+      rr.p0 = __builtin_amdgcn_readlane(prr.p0, k);
+      rr.p1 = __builtin_amdgcn_readlane(prr.p1, k);
+      rr.p2 = __builtin_amdgcn_readlane(prr.p2, k);
+      rr.pk = __builtin_amdgcn_readlane(prr.pk, k);
+      rr.w0 = __builtin_amdgcn_readlane(prr.w0, k);
+      rr.w1 = __builtin_amdgcn_readlane(prr.w1, k);
+      rr.w2 = __builtin_amdgcn_readlane(prr.w2, k);
+      setup = true;
+    }
+  }
+  return MPOA_OK;
+}
+
+// res_t entries from the logged moves of backtrack_wave (emit_sub of align.cpp:1603-1624):
+// a predecessor whose base is the query's is that node; otherwise the node aligned to its
+// column head with the query base, or a new node aligned to the head. All threads.
+template <int NT>
+__device__ void resolve_res(const Arena& A, const WinShared& S) {
+  const int beg = S.beg_i;
+  for (int x = threadIdx.x; x < S.m; x += NT) {
+    const int p = A.res_from[x];
+    const int qc = A.qseq[x];
+    if (p < 0) {
+      A.res_base[x] = (uint8_t)qc;
+      A.res_aligned[x] = -1;
+      continue;
+    }
+    const int pid = A.rank_id[beg + p];
+    const int pb = p == 0 ? CODE_N : A.base[pid];
+    if (pb == qc) {
+      A.res_from[x] = pid;
+      A.res_base[x] = A.base[pid];
+      A.res_aligned[x] = -1;
+    } else {
+      const int par = A.par[pid];
+      const int al = A.aligned[par * NB + qc];
+      A.res_from[x] = al;
+      A.res_base[x] = (uint8_t)qc;
+      A.res_aligned[x] = al != -1 ? -1 : par;
+    }
+  }
+}
+
+
 // Adaptive band of row i before its DP (src/align.cpp:1433-1444); one lane.
 __device__ void adaptive_band_row(const Arena& A, WinShared& S, int i, int w) {
   const int beg = S.beg_i, end = S.end_i, m = S.m;
@@ -1345,8 +1534,8 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
     }
   }
   __syncthreads();
+  const int64_t t2 = now();
   if (tid == 0) {
-    const int64_t t2 = now();
     S.ticks[PH_DP] += t2 - t1;
     S.cells += cells;
     S.edge_cells += ecells;
@@ -1354,8 +1543,20 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
     S.call_ecells = ecells;
     S.call_width = wmax;
     S.score = A.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)];
-    const int rc = backtrack(A, S, sc);
-    if (rc != MPOA_OK) S.status = rc;
+  }
+  if (__builtin_amdgcn_readfirstlane(tid >> 6) == 0) {  // wave 0 (a wave-uniform branch)
+    const int rc = backtrack_wave(A, S, sc);
+    if (tid == 0) {
+      // A row with more than 63 predecessors: the one-lane backtrack, which writes the
+      // res_t entries itself.
+      S.bt_serial = rc == 1;
+      S.bt_rc = rc == 1 ? backtrack(A, S, sc) : rc;
+    }
+  }
+  __syncthreads();
+  if (!S.bt_serial && S.bt_rc == MPOA_OK) resolve_res<NT>(A, S);
+  if (tid == 0) {
+    if (S.bt_rc != MPOA_OK) S.status = S.bt_rc;
     S.ticks[PH_BT] += now() - t2;
   }
   __syncthreads();
