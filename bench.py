@@ -237,7 +237,7 @@ def main() -> int:
     achieved = alg_bytes / (dom_ms / 1000.0) / 1e9
     kname = f"mpoa::k_windows<{dom.threads}, {dom.cols}>"
     traffic = measured_traffic(kname)
-    for L, ls in sorted(lst.items()):
+    for L, ls in sorted(lst.items()):  # 0 small, 1 medium, 2 re-run, 3 wide tile
         log(f"launch {L} ({ls.threads}x{ls.cols}, {ls.slots} slots): {ls.windows} windows, "
             f"{ls.cells / 1e9:.2f} G cells, {np.mean(launch_ms[L]):.1f} ms (HIP events)")
     line = {

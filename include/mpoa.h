@@ -113,11 +113,12 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
  * or 512) and `cols` DP columns per thread per pass (1, 2, 4 or 8). Not every pair is built;
  * returns MPOA_E_ARG for one that is not. Chosen by the autotuner (poa_amd/autotune.py). */
 int mpoa_batch_set_tile(mpoa_batch* b, int32_t threads, int32_t cols);
-/* Launch split: windows whose longest read is at most small_qmax and whose band is at most
- * small_width columns run on the small tile; of the others' launch, move_top (-1: half a
- * window per CU) of the costliest small-tile windows join it. Defaults 1500, 360, -1. */
+/* Launch split: windows whose longest read is at most small_qmax, whose band is at most
+ * small_width columns and whose graph is predicted to stay within small_nodes nodes run on
+ * the small tile; of the others' launch, move_top (-1: half a window per CU) of the costliest
+ * small-tile windows join it. Defaults 1500, 360, -1, no node limit. */
 int mpoa_batch_set_schedule(mpoa_batch* b, int32_t small_qmax, int32_t small_width,
-                            int32_t move_top);
+                            int32_t move_top, int32_t small_nodes);
 /* Runs the whole per-window pipeline on the device for every window; inputs are already
  * resident. Synchronous. */
 int mpoa_batch_execute(mpoa_batch* b);
@@ -132,7 +133,7 @@ void mpoa_batch_free(mpoa_batch* b);
  * sort's two parts: the order itself and hlen/tlen), in ticks
  * of the device wall clock (*tick_hz ticks per second). */
 typedef struct mpoa_window_stat {
-  int32_t status, nodes, edges, launch;  /* launch: 0 small-tile, 1 wide-tile, 2 re-run */
+  int32_t status, nodes, edges, launch;  /* launch: 0 small, 1 medium, 2 re-run, 3 wide tile */
   int64_t cells, edge_cells;
   int64_t ticks[8];
   int64_t t_begin, t_end;                /* device wall clock at the window's start and end */
@@ -163,7 +164,7 @@ int mpoa_batch_traces(mpoa_batch* b, mpoa_call_stat* out, int64_t cap, int64_t* 
 /* Per-launch record of the last execute: tile, resident workgroups, windows and DP cells the
  * launch processed, and its duration from HIP events recorded on the stream it ran on. */
 typedef struct mpoa_launch_stat {
-  int32_t launch;                        /* 0 small-tile, 1 wide-tile, 2 re-run */
+  int32_t launch;                        /* 0 small, 1 medium, 2 re-run, 3 wide tile */
   int32_t threads, cols, slots;
   int64_t windows, cells, edge_cells;
   double ms;

@@ -115,7 +115,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_launch_stats.argtypes = [VP, VP, I32, P(I32)]
         L.mpoa_batch_set_traces.argtypes = [VP, I32]
         L.mpoa_batch_set_mode.argtypes = [VP, I32]
-        L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32]
+        L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
         _lib = L
@@ -229,9 +229,10 @@ class Batch:
         return [arr[i] for i in range(self.n)], 1.0 / hz.value
 
     def set_schedule(self, small_qmax: int = 1500, small_width: int = 360,
-                     move_top: int = -1) -> None:
+                     move_top: int = -1, small_nodes: int = 1 << 30) -> None:
         """Launch split between the small tile and the large launch."""
-        _check(lib().mpoa_batch_set_schedule(self.h, small_qmax, small_width, move_top))
+        _check(lib().mpoa_batch_set_schedule(self.h, small_qmax, small_width, move_top,
+                                             small_nodes))
 
     def set_mode(self, mode: int) -> None:
         """MODE_BATCHED (persistent launches, work queue) or MODE_PLAIN (one launch per

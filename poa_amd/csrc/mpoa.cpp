@@ -33,7 +33,8 @@ namespace seed = mpoa::seed;
 struct mpoa_ctx {
   int device = 0;
   hipStream_t stream = nullptr;   // small-window launches, copies
-  hipStream_t stream2 = nullptr;  // large-window launches, concurrent with the first
+  hipStream_t stream2 = nullptr;  // medium-tile launches, concurrent with the first
+  hipStream_t stream3 = nullptr;  // wide-tile launches, concurrent with both
   int n_cu = 0;
   size_t mem_total = 0;
 };
@@ -190,12 +191,12 @@ struct mpoa_batch {
   DevBuf<WinDesc> d_win;
   DevBuf<WinResult> d_res;
   DevBuf<uint8_t> d_out;
-  Bin small, large, rerun;
-  std::vector<int> bin_of;    // launch of each window (0 small, 1 wide, 2 re-run)
+  Bin small, large, wide, rerun;  // large: the medium tile
+  std::vector<int> bin_of;    // launch of each window (0 small, 1 medium, 2 re-run, 3 wide)
   bool split = true;          // large windows in their own concurrent launch
   // schedule knobs (mpoa_batch_set_schedule): windows the small tile takes, and how many of
   // the costliest others join the large launch (-1: half a window per CU)
-  int small_qmax = 1500, small_width = 360, move_top = -1;
+  int small_qmax = 1500, small_width = 360, move_top = -1, small_nodes = 1 << 30;
   std::vector<WinResult> res;
   int64_t reruns = 0;
   double device_ms = 0;
@@ -389,8 +390,8 @@ int upload(mpoa_batch* b, int32_t n_windows, const int32_t* win_off, const char*
 void assign_bins(mpoa_batch* b) {
   b->small.wins.clear();
   b->large.wins.clear();
+  b->wide.wins.clear();
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
-  bool need_wide = !b->split;
   auto width_of = [&](const WinInfo& wi) {
     const int m = wi.qmax + 1;
     if (b->sc.f > 0 && !full) return 2 * (b->sc.b + m / b->sc.f) + 24;
@@ -399,15 +400,23 @@ void assign_bins(mpoa_batch* b) {
   for (int w : b->order) {
     const WinInfo& wi = b->info[w];
     const int width = b->pred_width.empty() ? width_of(wi) : (int)b->pred_width[w];
-    const bool large = !b->split || wi.qmax > b->small_qmax || width > b->small_width;
-    if (large && (wi.qmax > 4000 || width > 496)) need_wide = true;
-    (large ? b->large.wins : b->small.wins).push_back(w);
+    const int nodes = b->pred_nodes.empty() ? estimate(wi, b->sc, full).nodes
+                                            : (int)b->pred_nodes[w];
+    const bool large = !b->split || wi.qmax > b->small_qmax || width > b->small_width ||
+                       nodes > b->small_nodes;
+    // The wide tile for bands beyond the medium tile's ring and graphs beyond what its LDS
+    // sorts (about 4,300 nodes).
+    const bool wide = !b->split || width > 496 || nodes > 4200;
+    (!large ? b->small.wins : wide ? b->wide.wins : b->large.wins).push_back(w);
   }
-  b->large.nt = need_wide ? 256 : 128;
-  b->large.cpl = need_wide ? 2 : 4;
+  b->large.nt = 128;
+  b->large.cpl = 4;
+  // One window (minipoa's own run) takes the 512-thread tile: its rows are the whole time.
+  b->wide.nt = b->split ? 256 : 512;
+  b->wide.cpl = 2;
   b->bin_of.assign(b->n_windows, 0);
-  // The costliest remaining windows join the other launch up to half a window per CU: they
-  // set the makespan, and the larger tile shortens their DP rows and sorts their graphs in LDS.
+  // The costliest small-tile windows join the medium launch up to move_top windows (default
+  // half a window per CU): they set the makespan, and the larger tile shortens their rows.
   const size_t target = (size_t)std::max(0, b->move_top >= 0 ? b->move_top : b->ctx->n_cu / 2);
   if (b->split && b->large.wins.size() < target && !b->small.wins.empty()) {
     const size_t take = std::min(target - b->large.wins.size(), b->small.wins.size() / 8);
@@ -419,6 +428,7 @@ void assign_bins(mpoa_batch* b) {
     b->large.wins = merged;
   }
   for (int w : b->large.wins) b->bin_of[w] = 1;
+  for (int w : b->wide.wins) b->bin_of[w] = 3;
 }
 
 // Slot capacities for window w: the learned predictions when given (the caller includes its
@@ -484,7 +494,7 @@ int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vecto
     b->rerun.cpl = 2;
     b->rerun.wins = wins;
   }
-  for (Bin* bin : {&b->small, &b->large, &b->rerun}) {
+  for (Bin* bin : {&b->small, &b->large, &b->wide, &b->rerun}) {
     if (bin->wins.empty() || (worst != (bin == &b->rerun))) continue;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     if ((rc = provision(*bin, make_caps(est, b->info, bin->wins, keep_path), 1, free_b / 3)) !=
@@ -502,7 +512,10 @@ int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vecto
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, ctx->stream));
   for (int w : wins) {
-    const Bin& bin = worst ? b->rerun : b->bin_of[w] == 1 ? b->large : b->small;
+    const Bin& bin = worst              ? b->rerun
+                     : b->bin_of[w] == 1 ? b->large
+                     : b->bin_of[w] == 3 ? b->wide
+                                         : b->small;
     KernelArgs a{};
     a.scores = b->sc;
     a.caps = bin.caps;
@@ -794,7 +807,8 @@ int mpoa_create(int device, mpoa_ctx** out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
-      hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess) {
+      hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess ||
+      hipStreamCreate(&c->stream3) != hipSuccess) {
     delete c;
     g_err = "device initialisation failed";
     return MPOA_E_DEVICE;
@@ -809,6 +823,7 @@ void mpoa_destroy(mpoa_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
   delete ctx;
 }
 
@@ -921,6 +936,11 @@ int mpoa_batch_execute(mpoa_batch* b) {
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventCreate(&ev_large));
+  if (!b->wide.wins.empty()) {
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const SlotCaps c = make_caps(est, b->info, b->wide.wins, keep_path);
+    if ((rc = provision(b->wide, c, std::max(1, ctx->n_cu), free_b / 4)) != MPOA_OK) return rc;
+  }
   if (!b->large.wins.empty()) {
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     const SlotCaps c = make_caps(est, b->info, b->large.wins, keep_path);
@@ -932,21 +952,25 @@ int mpoa_batch_execute(mpoa_batch* b) {
     const int max_slots = std::max(1, ctx->n_cu * (b->small.nt == 64 ? 6 : 2));
     if ((rc = provision(b->small, c, max_slots, free_b / 4 * 3)) != MPOA_OK) return rc;
   }
-  b->first_slots = b->small.used_slots + b->large.used_slots;
+  b->first_slots = b->small.used_slots + b->large.used_slots + b->wide.used_slots;
   HIPCHK(hipEventRecord(ev0, ctx->stream));
   HIPCHK(hipStreamWaitEvent(ctx->stream2, ev0, 0));
+  HIPCHK(hipStreamWaitEvent(ctx->stream3, ev0, 0));
   // Large windows first so that they hold their CUs from the start: the small-window grid
   // (persistent, more workgroups than fit) would otherwise take every CU's LDS and keep the
   // large windows waiting until its queue drained. The host gives the large launch a short
   // head start before enqueueing the small one.
+  if ((rc = timed_enqueue(b, b->wide, ctx->stream3, 3, pend)) != MPOA_OK) return rc;
   if ((rc = timed_enqueue(b, b->large, ctx->stream2, 1, pend)) != MPOA_OK) return rc;
-  if (!b->large.wins.empty() && !b->small.wins.empty()) {
+  if ((!b->large.wins.empty() || !b->wide.wins.empty()) && !b->small.wins.empty()) {
     HIPCHK(hipStreamQuery(ctx->stream2) == hipErrorNotReady ? hipSuccess
                                                               : hipStreamQuery(ctx->stream2));
     std::this_thread::sleep_for(std::chrono::milliseconds(2));
   }
   if ((rc = timed_enqueue(b, b->small, ctx->stream, 0, pend)) != MPOA_OK) return rc;
   HIPCHK(hipEventRecord(ev_large, ctx->stream2));
+  HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
+  HIPCHK(hipEventRecord(ev_large, ctx->stream3));
   HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
   HIPCHK(hipEventRecord(ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ev1));
@@ -1023,7 +1047,7 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
     stats->launches = b->launches;
     stats->slots = b->first_slots;
     stats->pool_bytes = 0;
-    for (const Bin* bin : {&b->small, &b->large, &b->rerun})
+    for (const Bin* bin : {&b->small, &b->large, &b->wide, &b->rerun})
       stats->pool_bytes += (int64_t)(bin->d_int.n * sizeof(int) + bin->d_byte.n +
                                      bin->d_dp.n * sizeof(int));
   }
@@ -1187,11 +1211,12 @@ int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, i
 }
 
 int mpoa_batch_set_schedule(mpoa_batch* b, int32_t small_qmax, int32_t small_width,
-                            int32_t move_top) {
-  if (!b || small_qmax < 0 || small_width < 0) return MPOA_E_ARG;
+                            int32_t move_top, int32_t small_nodes) {
+  if (!b || small_qmax < 0 || small_width < 0 || small_nodes < 0) return MPOA_E_ARG;
   b->small_qmax = small_qmax;
   b->small_width = small_width;
   b->move_top = move_top;
+  b->small_nodes = small_nodes;
   assign_bins(b);
   return MPOA_OK;
 }

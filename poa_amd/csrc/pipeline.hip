@@ -385,11 +385,10 @@ __device__ __forceinline__ int block_excl_sum(int v, WinShared& S, int* total) {
 template <int NT>
 __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   const int N = S.n_nodes, E = S.n_out_e;
-  if (N > 32767 || E > 32767 || 2LL * (4LL * N + 2 + E) > lds_bytes) return false;
+  if (N > 32767 || E > 32767 || 2LL * (3LL * N + 1 + E) > lds_bytes) return false;
   int16_t* deg = reinterpret_cast<int16_t*>(lds);
   int16_t* stk = deg + N;
-  int16_t* rid = stk + N;
-  int16_t* ooff = rid + N;   // N + 1
+  int16_t* ooff = stk + N;   // N + 1
   int16_t* odst = ooff + N + 1;
   // out-degree prefix sums (node id order)
   int base = 0;
@@ -411,27 +410,70 @@ __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
     for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) odst[k++] = (int16_t)A.oe_dst[e];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  // The walk, by wave 0. Zero in-degree nodes go on the stack in ascending id order; each
+  // popped node's out-edges are processed by one lane each, in out-list order: the degree
+  // decrements (distinct targets: no two lanes touch one counter), then the targets that
+  // reached zero are pushed in edge order. The last one pushed is the next node popped, so it
+  // is kept in a register, with its out-edge range loaded alongside the decrements.
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
+    const int lane = threadIdx.x & 63;
     int sp = 0;
-    for (int v = 0; v < N; v++)
-      if (deg[v] == 0) stk[sp++] = (int16_t)v;
+    for (int v0 = 0; v0 < N; v0 += 64) {
+      const int v = v0 + lane;
+      const bool z = v < N && deg[v] == 0;
+      const uint64_t mk = __ballot(z);
+      if (z) stk[sp + __popcll(mk & ((1ULL << lane) - 1))] = (int16_t)v;
+      sp += __popcll(mk);
+    }
     int r = 0;
-    while (sp > 0) {
-      const int u = stk[--sp];
-      rid[r++] = (int16_t)u;
-      const int e1 = ooff[u + 1];
-      for (int e = ooff[u]; e < e1; e++) {
-        const int v = odst[e];
-        if (--deg[v] == 0) stk[sp++] = (int16_t)v;
+    int next = -1, next_o0 = 0, next_o1 = 0;
+    for (;;) {
+      int u, o0, o1;
+      if (next >= 0) {
+        u = next;
+        o0 = next_o0;
+        o1 = next_o1;
+      } else {
+        if (sp == 0) break;
+        u = stk[--sp];
+        o0 = ooff[u];
+        o1 = ooff[u + 1];
+      }
+      if (lane == 0) {
+        A.rank_id[r] = u;
+        A.rank[u] = r;
+      }
+      r++;
+      const int cnt = o1 - o0;
+      next = -1;
+      for (int b0 = 0; b0 < cnt; b0 += 64) {
+        const int k = b0 + lane;
+        int v = 0, vo0 = 0, vo1 = 0;
+        bool z = false;
+        if (k < cnt) {
+          v = odst[o0 + k];
+          const int d = deg[v] - 1;
+          deg[v] = (int16_t)d;
+          z = d == 0;
+          vo0 = ooff[v];
+          vo1 = ooff[v + 1];
+        }
+        const uint64_t mk = __ballot(z);
+        if (mk) {
+          // an earlier pending next goes under the new pushes
+          if (next >= 0 && lane == 0) stk[sp] = (int16_t)next;
+          if (next >= 0) sp++;
+          const int last = 63 - __clzll(mk);
+          const uint64_t rest = mk & ~(1ULL << last);
+          if (z && lane != last) stk[sp + __popcll(rest & ((1ULL << lane) - 1))] = (int16_t)v;
+          sp += __popcll(rest);
+          next = __builtin_amdgcn_readlane(v, last);
+          next_o0 = __builtin_amdgcn_readlane(vo0, last);
+          next_o1 = __builtin_amdgcn_readlane(vo1, last);
+        }
       }
     }
-    S.n_rows = r;
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < S.n_rows; r += NT) {
-    const int u = rid[r];
-    A.rank_id[r] = u;
-    A.rank[u] = r;
+    if (lane == 0) S.n_rows = r;
   }
   __syncthreads();
   return true;
@@ -1401,9 +1443,12 @@ __device__ void adaptive_update(const Arena& A, WinShared& S, int i, int max_j, 
 template <int NT>
 struct RingShape {
   static constexpr int RING = RING_ROWS;
-  static constexpr int WR = NT == 64 ? 368 : NT == 128 ? 512 : NT == 256 ? 1280 : 2048;
+  // 64/128/256: the batched tiles, sized so that the sort of a graph of up to ~3,000 /
+  // ~4,300 / ~7,000 nodes runs in LDS while leaving room for small-tile workgroups on the
+  // same CU (27 / 38 / 63 KB). 512: one window alone (minipoa's own run), wide bands.
+  static constexpr int WR = NT == 64 ? 368 : NT == 128 ? 512 : NT == 256 ? 896 : 2048;
   // longest query held in LDS
-  static constexpr int QCAP = NT == 64 ? 4 * WR + 64 : NT == 128 ? 4096 : 24576;
+  static constexpr int QCAP = NT == 64 ? 4 * WR + 64 : NT <= 256 ? 4096 : 24576;
 };
 
 // One poa() call on the device graph: rows [beg_i, end_i] of the current rank order.
@@ -1882,8 +1927,10 @@ __device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R
     }
     __syncthreads();
     if (S.status != MPOA_OK) break;
+    // The sort's LDS scratch is the whole ring: the query and row records are rewritten
+    // for every read.
     commit_read<NT>(a, A, S, r, m, path0, r + 1 == wd.n_reads, false,
-                    reinterpret_cast<uint8_t*>(R.H), (int)(sizeof(R.H) + sizeof(R.D)));
+                    reinterpret_cast<uint8_t*>(&R), (int)sizeof(R));
     path0 += qlen;
     __syncthreads();
     if (S.status != MPOA_OK) break;
