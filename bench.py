@@ -183,6 +183,7 @@ def main() -> int:
     prepare_s = time.perf_counter() - tp
     if args.nt or args.cpl:
         b.set_tile(args.nt or 64, args.cpl or 4)
+        b.tuning = {"small_tile": [args.nt or 64, args.cpl or 4], "from": "command line"}
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -291,6 +292,7 @@ def main() -> int:
         },
     }
     line["config"]["scheduling"] = learned or "read-length estimate"
+    line["config"]["tuning"] = b.tuning or "library defaults (no autotuner output)"
     if args.plain_sample:
         line["plain_one_window_per_launch"] = plain_form(ctx, params, batch, args.plain_sample,
                                                          res.texts, windows_per_s, log)

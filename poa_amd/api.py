@@ -200,7 +200,7 @@ class Batch:
     """Independent windows prepared on the device (mpoa_batch_prepare)."""
 
     def __init__(self, ctx: Context, params: Params, win_off: np.ndarray, seq: np.ndarray,
-                 seq_off: np.ndarray, size_hint: np.ndarray | None = None):
+                 seq_off: np.ndarray, size_hint: np.ndarray | None = None, tuned: bool = True):
         self.ctx = ctx
         self.win_off = np.ascontiguousarray(win_off, np.int32)
         seq = np.ascontiguousarray(seq, np.uint8)
@@ -212,6 +212,15 @@ class Batch:
                                         ctypes.byref(h)))
         self.h = h
         self.n = len(self.win_off) - 1
+        self.tuning = None
+        if tuned:
+            # the autotuner's choice for this device (poa_amd/autotune.py), when present
+            from . import autotune
+            t = autotune.load()
+            if t:
+                self.set_tile(*t["small_tile"])
+                self.set_schedule(move_top=t["move_top"])
+                self.tuning = {"small_tile": t["small_tile"], "move_top": t["move_top"]}
 
     def set_tile(self, threads: int, cols: int) -> None:
         _check(lib().mpoa_batch_set_tile(self.h, threads, cols))
