@@ -5,7 +5,7 @@ uneven windows drawn from a seed of its own (the same distribution as the 10,000
 workload), checks that every candidate's outputs are identical, and writes the fastest to
 poa_amd/tuned_gfx950.json, which ``api.Batch`` applies by default.
 
-    python -m poa_amd.autotune [--windows 3000] [--repeat 2]
+    python -m poa_amd.autotune [--windows 10000] [--repeat 2]
 """
 from __future__ import annotations
 
@@ -34,7 +34,7 @@ def load() -> dict | None:
 def main() -> int:
     from . import api, sched, synth
     ap = argparse.ArgumentParser()
-    ap.add_argument("--windows", type=int, default=3000)
+    ap.add_argument("--windows", type=int, default=10000)
     ap.add_argument("--repeat", type=int, default=2)
     ap.add_argument("--out", default=TUNED)
     args = ap.parse_args()
