@@ -864,15 +864,23 @@ __device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70
 
 // One predecessor's M/D-source contribution to this thread's columns (the band conditions of
 // src/align.cpp:1467 and :1484 as contiguous column ranges), from the LDS ring or from HBM.
+// The ring's row index and band of LDS slot k live in lane k of two registers (meta_row,
+// meta_bsbe; every wave keeps the same copy), so looking a predecessor up costs no LDS access.
+__device__ __forceinline__ bool ring_has(int meta_row, int i, int p, int ring) {
+  const int slot = __builtin_amdgcn_readfirstlane(p % ring);
+  return i - p < ring && __builtin_amdgcn_readlane(meta_row, slot) == p;
+}
+
 template <int CPL, class RingT>
 __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpScores& sc, int i,
                                              int p, int pbase, int cb, int c0, int c1,
-                                             const int* qc, int* h, int* d) {
+                                             const int* qc, int* h, int* d, int meta_row,
+                                             int meta_bsbe) {
   constexpr int RING = RingT::kRing, WR = RingT::kWR;
   if (p == 0) pbase = CODE_N;
-  const int slot = p % RING;
-  if (i - p < RING && R.row[slot] == p) {
-    const int bsbe = R.bsbe[slot];
+  const int slot = __builtin_amdgcn_readfirstlane(p % RING);
+  if (ring_has(meta_row, i, p, RING)) {
+    const int bsbe = __builtin_amdgcn_readlane(meta_bsbe, slot);
     const int bs = bsbe & 0xffff, be = bsbe >> 16;
     const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
     const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
@@ -901,7 +909,8 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
 // the number of in-window predecessors.
 template <int NT, int CPL, bool QLDS, class RingT>
 __device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
-                      const RowRec& rr, int bs, int be, bool want_argmax, int* np_in) {
+                      const RowRec& rr, int bs, int be, bool want_argmax, int* np_in,
+                      int& meta_row, int& meta_bsbe) {
   constexpr int RING = RingT::kRing, WR = RingT::kWR;
   const int tid = threadIdx.x;
   const int c0 = bs * Q, W = (be - bs - 1) * Q, c1 = c0 + W;
@@ -932,7 +941,8 @@ __device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc
       const int p = prank - beg;
       if (p < 0) continue;
       npi++;
-      pred_contrib<CPL>(A, R, sc, i, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d);
+      pred_contrib<CPL>(A, R, sc, i, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d,
+                        meta_row, meta_bsbe);
     }
     if (rr.np > 3) {
       for (int k = 3; k < rr.np; k++) {
@@ -941,7 +951,7 @@ __device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc
         drain_vmem();
         if (p < 0) continue;
         npi++;
-        pred_contrib<CPL>(A, R, sc, i, p, pbase, cb, c0, c1, qc, h, d);
+        pred_contrib<CPL>(A, R, sc, i, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe);
       }
       drain_vmem();
     }
@@ -980,9 +990,10 @@ __device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc
   *np_in = npi;
   // NEG_INF sentinel block after the band (align.cpp:1505-1506,1515).
   if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = NEG_INF;
-  if (tid == 0) {
-    R.row[slot_i] = cache_i ? i : -1;
-    R.bsbe[slot_i] = bs | (be << 16);
+  {
+    const int lane = tid & 63;
+    meta_row = lane == slot_i ? (cache_i ? i : -1) : meta_row;
+    meta_bsbe = lane == slot_i ? (bs | (be << 16)) : meta_bsbe;
   }
   if (!want_argmax) return 0;
   // First argmax over the row (value desc, column asc).
@@ -1485,6 +1496,8 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
     }
     __syncthreads();
   }
+  // Ring directory: lane k holds the row index (-1: none) and band of ring slot k.
+  int meta_row = -1, meta_bsbe = 0;
   // Row 0 (align.cpp:1415-1425): M, D = NEG_INF over blocks [Bs0, Be0) from the row start;
   // M[0][0] = 0. I of row 0 is never read; it is set to NEG_INF. Bs0 is always 0.
   {
@@ -1503,10 +1516,9 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
         R.D[j] = NEG_INF;
       }
     }
-    if (tid < RING) {
-      R.row[tid] = tid == 0 && cache0 ? 0 : -1;
-      R.bsbe[tid] = be0 << 16;
-    }
+    const int lane = tid & 63;
+    meta_row = lane == 0 && cache0 ? 0 : -1;
+    meta_bsbe = be0 << 16;
   }
   __syncthreads();
   const int64_t t1 = now();
@@ -1556,19 +1568,20 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
       const int n3 = min(rr.np, 3);
       for (int k = 0; k < n3; k++) {
         const int p = (k == 0 ? rr.p0 : k == 1 ? rr.p1 : rr.p2) - beg;
-        if (p >= 0) far |= !(i - p < RING && R.row[p % RING] == p);
+        if (p >= 0) far |= !ring_has(meta_row, i, p, RING);
       }
       if (rr.np > 3) {
         for (int k = 3; k < rr.np; k++) {
           const int p = A.pred_rank[rr.k0 + k] - beg;
           drain_vmem();
-          if (p >= 0) far |= !(i - p < RING && R.row[p % RING] == p);
+          if (p >= 0) far |= !ring_has(meta_row, i, p, RING);
         }
       }
       if (far) __syncthreads();
     }
     int np_in = 0;
-    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in);
+    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in,
+                                         meta_row, meta_bsbe);
     lds_barrier<NT>();
     cells += W;
     ecells += (int64_t)np_in * W;
