@@ -115,6 +115,22 @@ def measured_traffic(kernel: str):
     return None
 
 
+INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # CUs x lanes x peak engine clock
+
+
+def measured_int_ops(kernel: str):
+    """INT32 VALU lane operations per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (SQ_INSTS_VALU_INT32 x 64 lanes); None when it does not cover it."""
+    path = os.path.join(ROOT, "profiles", "k_windows_uneven10k_sq_counters.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("valu_int32_lane_ops_per_launch") and kernel in ln:
+                return int(ln.rsplit(None, 1)[-1])
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +307,12 @@ def main() -> int:
             "all_launches_ms": round(float(np.mean(kernel_ms)), 3),
         },
     }
+    int_ops = measured_int_ops(kname)
+    if int_ops is not None:
+        tops = int_ops / (dom_ms / 1e3) / 1e12
+        line["roofline"]["int32_valu"] = {
+            "achieved": round(tops, 3), "peak": INT32_VALU_PEAK_TOPS, "unit": "Tops/s",
+            "frac": round(tops / INT32_VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int_ops}
     line["config"]["scheduling"] = learned or "read-length estimate"
     line["config"]["tuning"] = b.tuning or "library defaults (no autotuner output)"
     if args.plain_sample:
