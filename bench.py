@@ -147,6 +147,8 @@ def main() -> int:
                     help="small_qmax,small_width,move_top for the launch split")
     ap.add_argument("--no-learned", action="store_true",
                     help="schedule by the read-length estimate instead of the trained network")
+    ap.add_argument("--eager", action="store_true",
+                    help="enqueue the launches one by one instead of replaying the hipGraph")
     ap.add_argument("--plain-sample", type=int, default=100,
                     help="windows run in the plain one-window-per-launch form (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -196,6 +198,7 @@ def main() -> int:
                    round(time.perf_counter() - tl, 3), "trained_on": model.meta}
     if args.schedule:
         b.set_schedule(*[int(x) for x in args.schedule.split(",")])
+    b.set_graph(not args.eager)
     prepare_s = time.perf_counter() - tp
     if args.nt or args.cpl:
         b.set_tile(args.nt or 64, args.cpl or 4)
@@ -315,6 +318,9 @@ def main() -> int:
             "frac": round(tops / INT32_VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int_ops}
     line["config"]["scheduling"] = learned or "read-length estimate"
     line["config"]["tuning"] = b.tuning or "library defaults (no autotuner output)"
+    line["config"]["launch_form"] = ("eager launches on three streams" if args.eager else
+                                     f"hipGraph replay ({st.graph_builds} build, "
+                                     f"{st.graph_launches} launch per step)")
     if args.plain_sample:
         line["plain_one_window_per_launch"] = plain_form(ctx, params, batch, args.plain_sample,
                                                          res.texts, windows_per_s, log)

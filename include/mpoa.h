@@ -100,6 +100,9 @@ typedef struct mpoa_batch_stats {
   int32_t launches;      /* pipeline launches in the last execute (1 + re-run passes) */
   int32_t slots;         /* resident workgroups of the first launch */
   int64_t pool_bytes;    /* device memory held by the per-slot arenas */
+  int32_t graph_launches; /* graph replays in the last execute (0: eager launches) */
+  int32_t graph_builds;   /* graphs built over the batch's life (a rebuild follows any change
+                             of a launch's arguments) */
 } mpoa_batch_stats;
 
 /* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window
@@ -153,6 +156,13 @@ int mpoa_batch_set_predictions(mpoa_batch* b, const float* pred, int32_t stride)
  * another in input order (the comparison form). Results are identical. */
 enum { MPOA_MODE_BATCHED = 0, MPOA_MODE_PLAIN = 1 };
 int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
+
+/* Graph replay of the batched pass (default on): the concurrent persistent launches, each with
+ * its work-queue reset, are captured once into a hipGraph and replayed by every later
+ * mpoa_batch_execute whose launch arguments are unchanged; off, they are enqueued one by one.
+ * Overflow re-runs are enqueued eagerly in both. Results are identical. No counterpart in
+ * minipoa, which has no device launches. */
+int mpoa_batch_set_graph(mpoa_batch* b, int32_t enable);
 
 /* Per-DP-call traces: when enabled before mpoa_batch_execute, every DP call records its
  * geometry (rows, query length, band extent, cells, graph size) and score. mpoa_batch_traces

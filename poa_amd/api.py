@@ -27,7 +27,8 @@ EXPORTS = [
     "mpoa_create", "mpoa_destroy", "mpoa_strerror", "mpoa_align_reads", "mpoa_batch_prepare",
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
-    "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_predictions",
+    "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph",
+    "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
     "mpoa_mlp_train", "mpoa_mlp_error",
@@ -85,7 +86,8 @@ class BatchStats(ctypes.Structure):
                 ("cells", ctypes.c_int64), ("edge_cells", ctypes.c_int64),
                 ("reruns", ctypes.c_int64), ("device_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int32),
-                ("slots", ctypes.c_int32), ("pool_bytes", ctypes.c_int64)]
+                ("slots", ctypes.c_int32), ("pool_bytes", ctypes.c_int64),
+                ("graph_launches", ctypes.c_int32), ("graph_builds", ctypes.c_int32)]
 
 
 _lib = None
@@ -115,6 +117,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_launch_stats.argtypes = [VP, VP, I32, P(I32)]
         L.mpoa_batch_set_traces.argtypes = [VP, I32]
         L.mpoa_batch_set_mode.argtypes = [VP, I32]
+        L.mpoa_batch_set_graph.argtypes = [VP, I32]
         L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
@@ -247,6 +250,11 @@ class Batch:
         """MODE_BATCHED (persistent launches, work queue) or MODE_PLAIN (one launch per
         window, in input order)."""
         _check(lib().mpoa_batch_set_mode(self.h, mode))
+
+    def set_graph(self, enable: bool) -> None:
+        """Replay the batched pass as a captured hipGraph (default) or enqueue its launches
+        one by one."""
+        _check(lib().mpoa_batch_set_graph(self.h, int(bool(enable))))
 
     def set_predictions(self, pred: np.ndarray) -> None:
         """Per-window (cost, max call cells, max band width, final nodes) predictions that

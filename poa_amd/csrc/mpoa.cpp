@@ -37,6 +37,7 @@ struct mpoa_ctx {
   hipStream_t stream3 = nullptr;  // wide-tile launches, concurrent with both
   int n_cu = 0;
   size_t mem_total = 0;
+  int wall_khz = 100000;          // device wall clock (the k_hold timer)
 };
 
 namespace {
@@ -213,6 +214,22 @@ struct mpoa_batch {
   DevBuf<CallStat> d_calls;
   DevBuf<int64_t> d_call_off;
   std::vector<int64_t> call_off;  // per window: first call record
+  // Graph form of the batched pass (mpoa_batch_set_graph): per launch an event record, a
+  // queue reset, the persistent kernel and an event record, the three launches as parallel
+  // branches. Rebuilt when a launch's arguments change; replayed otherwise.
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> graph_key;
+  hipEvent_t gev[6] = {};
+  int graph_builds = 0;
+  int graph_launches = 0;  // in the last execute
+  ~mpoa_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : gev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
@@ -278,16 +295,19 @@ int provision(Bin& bin, const SlotCaps& caps, int max_slots, size_t budget) {
   return MPOA_OK;
 }
 
-// Enqueues one persistent launch of a provisioned bin on `stream` (no wait).
-int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
-  if (bin.wins.empty()) return MPOA_OK;
+// Uploads a bin's processing order on `stream` (allocating its order and queue buffers).
+int upload_order(Bin& bin, hipStream_t stream) {
   int rc;
   if (!bin.d_order.p || bin.d_order.n < bin.wins.size())
     if ((rc = bin.d_order.alloc(bin.wins.size())) != MPOA_OK) return rc;
   if (!bin.d_queue.p && (rc = bin.d_queue.alloc(1)) != MPOA_OK) return rc;
   HIPCHK(hipMemcpyAsync(bin.d_order.p, bin.wins.data(), bin.wins.size() * sizeof(int),
                         hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemsetAsync(bin.d_queue.p, 0, sizeof(int), stream));
+  return MPOA_OK;
+}
+
+// The persistent kernel's arguments for a provisioned bin.
+KernelArgs args_of(const mpoa_batch* b, const Bin& bin) {
   KernelArgs a{};
   a.scores = b->sc;
   a.caps = bin.caps;
@@ -305,6 +325,16 @@ int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
   a.call_stats = b->d_calls.p;
   a.call_off = b->d_call_off.p;
   a.gfa_lines = b->d_gfa.p;
+  return a;
+}
+
+// Enqueues one persistent launch of a provisioned bin on `stream` (no wait).
+int enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream) {
+  if (bin.wins.empty()) return MPOA_OK;
+  int rc;
+  if ((rc = upload_order(bin, stream)) != MPOA_OK) return rc;
+  HIPCHK(hipMemsetAsync(bin.d_queue.p, 0, sizeof(int), stream));
+  const KernelArgs a = args_of(b, bin);
   const int lrc = mpoa_launch_windows(a, bin.used_slots, bin.nt, bin.cpl, stream);
   if (lrc != 0) {
     g_err = lrc == -2 ? "unsupported tile shape" : "kernel launch failed";
@@ -320,6 +350,7 @@ struct Pending {
   const Bin* bin;
   std::vector<int> wins;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool owned = true;  // events created for this launch (the graph's are the batch's)
 };
 
 int timed_enqueue(mpoa_batch* b, Bin& bin, hipStream_t stream, int kind,
@@ -356,10 +387,87 @@ int close_pending(mpoa_batch* b, std::vector<Pending>& pend, const std::vector<W
     }
     ls.ms = ms;
     b->lstats.push_back(ls);
-    (void)hipEventDestroy(p.e0);
-    (void)hipEventDestroy(p.e1);
+    if (p.owned) {
+      (void)hipEventDestroy(p.e0);
+      (void)hipEventDestroy(p.e1);
+    }
   }
   pend.clear();
+  return MPOA_OK;
+}
+
+// The batched pass as one graph launch on the context's stream: the wide, medium and small
+// launches as parallel branches (queue reset, persistent kernel, an event record on either
+// side), the small one held 2 ms so that the larger tiles hold their CUs first.
+int graph_pass(mpoa_batch* b, std::vector<Pending>& pend) {
+  mpoa_ctx* ctx = b->ctx;
+  Bin* bins[3] = {&b->wide, &b->large, &b->small};
+  const int kinds[3] = {3, 1, 0};
+  std::vector<unsigned char> key;
+  int rc;
+  for (int k = 0; k < 3; k++) {
+    Bin& bin = *bins[k];
+    if (bin.wins.empty()) continue;
+    if ((rc = upload_order(bin, ctx->stream)) != MPOA_OK) return rc;
+    const KernelArgs a = args_of(b, bin);
+    const int geo[4] = {k, bin.used_slots, bin.nt, bin.cpl};
+    const unsigned char* pa = reinterpret_cast<const unsigned char*>(&a);
+    const unsigned char* pg = reinterpret_cast<const unsigned char*>(geo);
+    key.insert(key.end(), pa, pa + sizeof a);
+    key.insert(key.end(), pg, pg + sizeof geo);
+  }
+  if (key.empty()) return MPOA_OK;
+  if (!b->gexec || key != b->graph_key) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_key.clear();
+    for (hipEvent_t& e : b->gev)
+      if (!e) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipGraphCreate(&b->graph, 0));
+    const bool others = !b->wide.wins.empty() || !b->large.wins.empty();
+    for (int k = 0; k < 3; k++) {
+      Bin& bin = *bins[k];
+      if (bin.wins.empty()) continue;
+      hipGraphNode_t head = nullptr, e0, kw, e1;
+      size_t n_head = 0;
+      if (kinds[k] == 0 && others) {
+        if (mpoa_graph_add_hold(b->graph, nullptr, 0, (int64_t)ctx->wall_khz * 2, &head) != 0) {
+          g_err = "graph construction failed";
+          return MPOA_E_DEVICE;
+        }
+        n_head = 1;
+      }
+      HIPCHK(hipGraphAddEventRecordNode(&e0, b->graph, n_head ? &head : nullptr, n_head,
+                                        b->gev[2 * k]));
+      const int grc = mpoa_graph_add_windows(b->graph, &e0, 1, args_of(b, bin), bin.used_slots,
+                                             bin.nt, bin.cpl, &kw);
+      if (grc != 0) {
+        g_err = grc == -2 ? "unsupported tile shape" : "graph construction failed";
+        return grc == -2 ? MPOA_E_ARG : MPOA_E_DEVICE;
+      }
+      HIPCHK(hipGraphAddEventRecordNode(&e1, b->graph, &kw, 1, b->gev[2 * k + 1]));
+    }
+    HIPCHK(hipGraphInstantiateWithFlags(&b->gexec, b->graph, 0));
+    b->graph_key = key;
+    b->graph_builds++;
+  }
+  HIPCHK(hipGraphLaunch(b->gexec, ctx->stream));
+  b->graph_launches++;
+  for (int k = 0; k < 3; k++) {
+    Bin& bin = *bins[k];
+    if (bin.wins.empty()) continue;
+    Pending p;
+    p.kind = kinds[k];
+    p.bin = &bin;
+    p.wins = bin.wins;
+    p.e0 = b->gev[2 * k];
+    p.e1 = b->gev[2 * k + 1];
+    p.owned = false;
+    pend.push_back(std::move(p));
+    b->launches++;
+  }
   return MPOA_OK;
 }
 
@@ -815,6 +923,10 @@ int mpoa_create(int device, mpoa_ctx** out) {
   }
   c->n_cu = prop.multiProcessorCount;
   c->mem_total = prop.totalGlobalMem;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+      khz > 0)
+    c->wall_khz = khz;
   *out = c;
   return MPOA_OK;
 }
@@ -920,6 +1032,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
   b->reruns = 0;
   b->kernel_ms = 0;
   b->launches = 0;
+  b->graph_launches = 0;
   b->lstats.clear();
   if (n == 0) return MPOA_OK;
   std::vector<Pending> pend;
@@ -953,6 +1066,10 @@ int mpoa_batch_execute(mpoa_batch* b) {
     if ((rc = provision(b->small, c, max_slots, free_b / 4 * 3)) != MPOA_OK) return rc;
   }
   b->first_slots = b->small.used_slots + b->large.used_slots + b->wide.used_slots;
+  if (b->use_graph) {
+    HIPCHK(hipEventRecord(ev0, ctx->stream));
+    if ((rc = graph_pass(b, pend)) != MPOA_OK) return rc;
+  } else {
   HIPCHK(hipEventRecord(ev0, ctx->stream));
   HIPCHK(hipStreamWaitEvent(ctx->stream2, ev0, 0));
   HIPCHK(hipStreamWaitEvent(ctx->stream3, ev0, 0));
@@ -972,6 +1089,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
   HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
   HIPCHK(hipEventRecord(ev_large, ctx->stream3));
   HIPCHK(hipStreamWaitEvent(ctx->stream, ev_large, 0));
+  }
   HIPCHK(hipEventRecord(ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ev1));
   float kms = 0;
@@ -1046,6 +1164,8 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
     stats->kernel_ms = b->kernel_ms;
     stats->launches = b->launches;
     stats->slots = b->first_slots;
+    stats->graph_launches = b->graph_launches;
+    stats->graph_builds = b->graph_builds;
     stats->pool_bytes = 0;
     for (const Bin* bin : {&b->small, &b->large, &b->wide, &b->rerun})
       stats->pool_bytes += (int64_t)(bin->d_int.n * sizeof(int) + bin->d_byte.n +
@@ -1108,6 +1228,12 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
     o.edge_cells = r.edge_cells;
     for (int k = 0; k < 8; k++) o.ticks[k] = r.ticks[k];
   }
+  return MPOA_OK;
+}
+
+int mpoa_batch_set_graph(mpoa_batch* b, int32_t enable) {
+  if (!b) return MPOA_E_ARG;
+  b->use_graph = enable != 0;
   return MPOA_OK;
 }
 

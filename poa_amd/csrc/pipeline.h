@@ -123,6 +123,14 @@ struct SeedArgs {
 int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
                         hipStream_t stream);
 bool mpoa_tile_supported(int nt, int cpl);
+// Graph form of one persistent launch: a node resetting the work queue after `deps`, then the
+// k_windows node (returned in *last). 0, -1 on a graph error, -2 for an unsupported tile.
+int mpoa_graph_add_windows(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
+                           const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
+                           hipGraphNode_t* last);
+// A one-wavefront node that waits `ticks` of the device wall clock after `deps`.
+int mpoa_graph_add_hold(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps, int64_t ticks,
+                        hipGraphNode_t* node);
 // Seeded windows: every sub-window of one read (one workgroup each), then the commit of that
 // read (one workgroup). 0, -1 on a launch error, -2 for an unsupported tile.
 int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream);

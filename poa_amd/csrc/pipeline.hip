@@ -2125,6 +2125,20 @@ __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
   }
 }
 
+// Resets a work-queue counter (a kernel rather than a memset: it goes into graphs).
+__global__ void k_fill(int* p, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Holds one wavefront for `ticks` of the wall clock (at most 65,536 sleeps of 64 x 64 cycles,
+// ~0.1 s): in a graph it delays the launch that follows it on its branch, the way a host sleep
+// does between eager launches.
+__global__ void k_hold(int64_t ticks) {
+  const int64_t t0 = now();
+  for (int i = 0; i < (1 << 16) && now() - t0 < ticks; i++) __builtin_amdgcn_s_sleep(64);
+}
+
 }  // namespace mpoa
 
 #define MPOA_TILES(X) \
@@ -2149,6 +2163,47 @@ int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
   MPOA_TILES(MPOA_L)
 #undef MPOA_L
   return -2;
+}
+
+int mpoa_graph_add_windows(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
+                           const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
+                           hipGraphNode_t* last) {
+  using namespace mpoa;
+  const void* fn = nullptr;
+#define MPOA_G(NT_, CPL_) \
+  if (nt == NT_ && cpl == CPL_) fn = reinterpret_cast<const void*>(&k_windows<NT_, CPL_>);
+  MPOA_TILES(MPOA_G)
+#undef MPOA_G
+  if (!fn) return -2;
+  int* queue = a.queue;
+  int zero = 0, one = 1;
+  void* fill_args[] = {&queue, &zero, &one};
+  hipKernelNodeParams fp{};
+  fp.func = reinterpret_cast<void*>(&k_fill);
+  fp.gridDim = dim3(1);
+  fp.blockDim = dim3(64);
+  fp.kernelParams = fill_args;
+  hipGraphNode_t fill;
+  if (hipGraphAddKernelNode(&fill, g, deps, n_deps, &fp) != hipSuccess) return -1;
+  KernelArgs args = a;
+  void* win_args[] = {&args};
+  hipKernelNodeParams wp{};
+  wp.func = const_cast<void*>(fn);
+  wp.gridDim = dim3(n_slots);
+  wp.blockDim = dim3(nt);
+  wp.kernelParams = win_args;
+  return hipGraphAddKernelNode(last, g, &fill, 1, &wp) == hipSuccess ? 0 : -1;
+}
+
+int mpoa_graph_add_hold(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps, int64_t ticks,
+                        hipGraphNode_t* node) {
+  void* args[] = {&ticks};
+  hipKernelNodeParams p{};
+  p.func = reinterpret_cast<void*>(&mpoa::k_hold);
+  p.gridDim = dim3(1);
+  p.blockDim = dim3(64);
+  p.kernelParams = args;
+  return hipGraphAddKernelNode(node, g, deps, n_deps, &p) == hipSuccess ? 0 : -1;
 }
 
 int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream) {
