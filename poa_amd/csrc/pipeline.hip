@@ -118,6 +118,7 @@ struct WinShared {
   int call_width;                   // widest row of the last DP call
   int bt_rc;                        // backtrack result of the last DP call
   bool bt_serial;                   // it ran on one lane
+  int kahn_over;                    // the LDS sort's stack would have overflowed
   int64_t total;  // DP storage of the current call
   int job;
   int scan[16];
@@ -380,31 +381,38 @@ __device__ __forceinline__ int block_excl_sum(int v, WinShared& S, int* total) {
   return before + incl - v;
 }
 
-// Kahn order into A.rank_id[0..N) using LDS (lds_bytes available at lds). Returns false when
-// the graph does not fit.
+// Stack capacity of the LDS walk: the stack holds the released nodes not yet popped, a few
+// dozen in an alignment graph; a walk that would exceed it starts over on the HBM arrays.
+constexpr int KAHN_STK = 512;
+
+// Kahn order into A.rank_id[0..N) using LDS (lds_bytes available at lds): out-edge CSR as
+// int16 (offsets N + 1, targets E), in-degrees as bytes, a KAHN_STK-entry stack. Returns false
+// when the graph does not fit, an in-degree exceeds 255 or the stack would overflow.
 template <int NT>
 __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   const int N = S.n_nodes, E = S.n_out_e;
-  if (N > 32767 || E > 32767 || 2LL * (3LL * N + 1 + E) > lds_bytes) return false;
-  int16_t* deg = reinterpret_cast<int16_t*>(lds);
-  int16_t* stk = deg + N;
-  int16_t* ooff = stk + N;   // N + 1
+  if (N > 32767 || E > 32767 || 2LL * (N + 1 + E + KAHN_STK) + N > lds_bytes) return false;
+  int16_t* ooff = reinterpret_cast<int16_t*>(lds);  // N + 1
   int16_t* odst = ooff + N + 1;
+  int16_t* stk = odst + E;
+  uint8_t* deg = reinterpret_cast<uint8_t*>(stk + KAHN_STK);
   // out-degree prefix sums (node id order)
-  int base = 0;
+  int base = 0, wide = 0;
   for (int v0 = 0; v0 < N; v0 += NT) {
     const int v = v0 + threadIdx.x;
     const int oc = v < N ? A.out_cnt[v] : 0;
     int tot;
     const int ex = block_excl_sum<NT>(oc, S, &tot);
     if (v < N) {
+      const int ic = A.in_cnt[v];
+      wide |= ic > 255;
       ooff[v] = (int16_t)(base + ex);
-      deg[v] = (int16_t)A.in_cnt[v];
+      deg[v] = (uint8_t)ic;
     }
     base += tot;
   }
   if (threadIdx.x == 0) ooff[N] = (int16_t)base;
-  __syncthreads();
+  if (__syncthreads_or(wide)) return false;
   for (int v = threadIdx.x; v < N; v += NT) {
     int k = ooff[v];
     for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) odst[k++] = (int16_t)A.oe_dst[e];
@@ -418,16 +426,18 @@ __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
     const int lane = threadIdx.x & 63;
     int sp = 0;
-    for (int v0 = 0; v0 < N; v0 += 64) {
+    bool over = false;
+    for (int v0 = 0; v0 < N && !over; v0 += 64) {
       const int v = v0 + lane;
       const bool z = v < N && deg[v] == 0;
       const uint64_t mk = __ballot(z);
-      if (z) stk[sp + __popcll(mk & ((1ULL << lane) - 1))] = (int16_t)v;
+      over = sp + __popcll(mk) > KAHN_STK;
+      if (z && !over) stk[sp + __popcll(mk & ((1ULL << lane) - 1))] = (int16_t)v;
       sp += __popcll(mk);
     }
     int r = 0;
     int next = -1, next_o0 = 0, next_o1 = 0;
-    for (;;) {
+    while (!over) {
       int u, o0, o1;
       if (next >= 0) {
         u = next;
@@ -453,12 +463,16 @@ __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
         if (k < cnt) {
           v = odst[o0 + k];
           const int d = deg[v] - 1;
-          deg[v] = (int16_t)d;
+          deg[v] = (uint8_t)d;
           z = d == 0;
           vo0 = ooff[v];
           vo1 = ooff[v + 1];
         }
         const uint64_t mk = __ballot(z);
+        if (sp + __popcll(mk) > KAHN_STK) {
+          over = true;
+          break;
+        }
         if (mk) {
           // an earlier pending next goes under the new pushes
           if (next >= 0 && lane == 0) stk[sp] = (int16_t)next;
@@ -473,10 +487,13 @@ __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
         }
       }
     }
-    if (lane == 0) S.n_rows = r;
+    if (lane == 0) {
+      S.n_rows = r;
+      S.kahn_over = over;
+    }
   }
   __syncthreads();
-  return true;
+  return !S.kahn_over;
 }
 
 // Kahn order on the HBM arrays (one lane), for graphs whose adjacency does not fit in LDS.
@@ -534,6 +551,47 @@ __device__ void tree_depth(int n, const int* par, int* depth, int* jmp, int base
   __syncthreads();
 }
 
+// The same depths with the forest in LDS (int16 parents, -1 at roots; UP: parents have smaller
+// ranks). Blocks of 64 ranks in parent-first order: a lane whose parent lies in an earlier
+// block reads that parent's depth; links to parents inside the block are resolved by pointer
+// jumping across the lanes (Wyllie's list ranking: at most six steps for 63 links). dep is the
+// LDS copy of the depths the later blocks read; out gets base + depth. Wave 0 only: within one
+// wavefront LDS operations complete in issue order, so the next block sees every store. Not
+// inlined: inside the window kernel it would push the register allocation into scratch.
+template <bool UP>
+__device__ __noinline__ void tree_depth_lds(int n, const int16_t* par, int16_t* dep, int* out, int base) {
+  const int lane = threadIdx.x & 63;
+  for (int b = 0; b < n; b += 64) {
+    const int hi = n - 1 - b;  // highest rank of the block when parents have larger ranks
+    const int r = UP ? b + lane : hi - lane;
+    const bool act = b + lane < n;
+    const int p = act ? par[r] : -1;
+    int d = 0, link = lane, done = 1;
+    if (p >= 0) {
+      if (UP ? p >= b : p <= hi) {
+        d = 1;
+        link = UP ? p - b : hi - p;
+        done = 0;
+      } else {
+        d = dep[p] + 1;
+      }
+    }
+    for (int step = 0; step < 7 && !__all(done); step++) {
+      const int dl = __shfl(d, link, 64);
+      const int donel = __shfl(done, link, 64);
+      const int ll = __shfl(link, link, 64);
+      if (!done) {
+        d += dl;
+        if (donel) done = 1; else link = ll;
+      }
+    }
+    if (act) {
+      dep[r] = (int16_t)d;
+      out[r] = base + d;
+    }
+  }
+}
+
 template <int NT>
 __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
                          int lds_bytes) {
@@ -559,6 +617,11 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
   __syncthreads();
   int* hpar = A.t0;  // max-in-weight predecessor rank (hlen tree)
   int* tchd = A.t1;  // max-out-weight successor rank (tlen tree)
+  // the two trees (and the depths) in LDS when they fit; the sort's LDS arrays are dead here
+  const bool lds_len = with_len && n <= 32767 && 6LL * n <= lds_bytes;
+  int16_t* lpar = reinterpret_cast<int16_t*>(lds);
+  int16_t* lchd = lpar + n;
+  int16_t* ldep = lchd + n;
   for (int r = threadIdx.x; r < n; r += NT) {
     const int u = A.rank_id[r];
     const int k0 = A.pred_off[r];
@@ -593,14 +656,27 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
     rec[8] = w3[0];
     rec[9] = w3[1];
     rec[10] = w3[2];
-    hpar[r] = max_pre;
-    tchd[r] = max_suc;
+    if (lds_len) {
+      lpar[r] = (int16_t)max_pre;
+      lchd[r] = (int16_t)max_suc;
+    } else {
+      hpar[r] = max_pre;
+      tchd[r] = max_suc;
+    }
   }
   __syncthreads();
   if (with_len) {
     const int64_t td = now();
-    tree_depth<NT>(n, hpar, A.hlen, A.t2, 0);
-    tree_depth<NT>(n, tchd, A.tlen, A.t2, 1);
+    if (lds_len) {
+      if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
+        tree_depth_lds<true>(n, lpar, ldep, A.hlen, 0);
+        tree_depth_lds<false>(n, lchd, ldep, A.tlen, 1);
+      }
+      __syncthreads();
+    } else {
+      tree_depth<NT>(n, hpar, A.hlen, A.t2, 0);
+      tree_depth<NT>(n, tchd, A.tlen, A.t2, 1);
+    }
     if (threadIdx.x == 0) S.ticks[PH_DEPTH] += now() - td;
   }
 }
