@@ -240,7 +240,7 @@ class Batch:
         _check(lib().mpoa_batch_window_stats(self.h, arr, self.n, ctypes.byref(hz)))
         return [arr[i] for i in range(self.n)], 1.0 / hz.value
 
-    def set_schedule(self, small_qmax: int = 1500, small_width: int = 360,
+    def set_schedule(self, small_qmax: int = 1500, small_width: int = 344,
                      move_top: int = -1, small_nodes: int = 1 << 30) -> None:
         """Launch split between the small tile and the large launch."""
         _check(lib().mpoa_batch_set_schedule(self.h, small_qmax, small_width, move_top,

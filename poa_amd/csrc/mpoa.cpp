@@ -197,7 +197,7 @@ struct mpoa_batch {
   bool split = true;          // large windows in their own concurrent launch
   // schedule knobs (mpoa_batch_set_schedule): windows the small tile takes, and how many of
   // the costliest others join the large launch (-1: half a window per CU)
-  int small_qmax = 1500, small_width = 360, move_top = -1, small_nodes = 1 << 30;
+  int small_qmax = 1500, small_width = 344, move_top = -1, small_nodes = 1 << 30;
   std::vector<WinResult> res;
   int64_t reruns = 0;
   double device_ms = 0;
@@ -1062,7 +1062,8 @@ int mpoa_batch_execute(mpoa_batch* b) {
   if (!b->small.wins.empty()) {
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     const SlotCaps c = make_caps(est, b->info, b->small.wins, keep_path);
-    const int max_slots = std::max(1, ctx->n_cu * (b->small.nt == 64 ? 6 : 2));
+    const int per_cu = mpoa_tile_occupancy(b->small.nt, b->small.cpl);
+    const int max_slots = std::max(1, ctx->n_cu * std::max(1, per_cu));
     if ((rc = provision(b->small, c, max_slots, free_b / 4 * 3)) != MPOA_OK) return rc;
   }
   b->first_slots = b->small.used_slots + b->large.used_slots + b->wide.used_slots;

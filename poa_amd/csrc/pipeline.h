@@ -123,6 +123,8 @@ struct SeedArgs {
 int mpoa_launch_windows(const mpoa::KernelArgs& a, int n_slots, int nt, int cpl,
                         hipStream_t stream);
 bool mpoa_tile_supported(int nt, int cpl);
+// Workgroups of the (nt, cpl) window kernel one CU holds at once (0 if unsupported).
+int mpoa_tile_occupancy(int nt, int cpl);
 // Graph form of one persistent launch: a node resetting the work queue after `deps`, then the
 // k_windows node (returned in *last). 0, -1 on a graph error, -2 for an unsupported tile.
 int mpoa_graph_add_windows(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,

@@ -42,10 +42,11 @@ __device__ __forceinline__ uint8_t char_of(int code) {
   return (uint8_t)(code == 0 ? 'A' : code == 1 ? 'C' : code == 2 ? 'G' : code == 3 ? 'T' : 'N');
 }
 
-// Per-slot arena: carved from the int pool / byte pool / DP pool at fixed offsets.
-// Row-record flag: a successor reads the row back from HBM.
-constexpr int RR_FAR = 1 << 12;
-constexpr int RING_ROWS = 8;  // rows kept in the LDS ring (every tile)
+// Per-slot arena: carved from the int pool / byte pool / DP pool at fixed offsets. Every
+// device function that takes the arena is force-inlined: one left as a call needs the arena's
+// address, which moves the whole structure to scratch memory and turns each of its pointers
+// into a scratch load inside the DP, sort and backtrack loops.
+constexpr int RING_ROWS = 8;  // rows kept in the LDS ring (tiles above 64 threads)
 
 struct Arena {
   uint8_t* base;
@@ -134,7 +135,7 @@ __device__ __forceinline__ int64_t now() { return (int64_t)wall_clock64(); }
 // Graph edits (one lane). Follow graph::add_adj / node_t::add_in_adj / add_out_adj
 // (src/graph.cpp:13-20, include/node.h:23-61) and graph::add_path (graph.cpp:170-235).
 // ---------------------------------------------------------------------------------------
-__device__ int new_node(Arena& A, WinShared& S, const SlotCaps& c, int base) {
+__device__ __forceinline__ int new_node(Arena& A, WinShared& S, const SlotCaps& c, int base) {
   int id = S.n_nodes;
   if (id >= c.nodes) { S.status = MPOA_E_OVERFLOW; return -1; }
   S.n_nodes = id + 1;
@@ -150,7 +151,7 @@ __device__ int new_node(Arena& A, WinShared& S, const SlotCaps& c, int base) {
   return id;
 }
 
-__device__ void add_adj(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path, int seq_id,
+__device__ __forceinline__ void add_adj(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path, int seq_id,
                         int from, int to, int pos, int path0) {
   if (from == to) return;
   // out-list of `from`
@@ -190,7 +191,7 @@ __device__ void add_adj(Arena& A, WinShared& S, const SlotCaps& c, bool keep_pat
   }
 }
 
-__device__ void add_path(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path, int seq_id,
+__device__ __forceinline__ void add_path(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path, int seq_id,
                          int m, int path0) {
   int anchor = -1, pos = 0;
   for (int r = 0; r < m && S.status == MPOA_OK; r++) {
@@ -225,7 +226,7 @@ __device__ void add_path(Arena& A, WinShared& S, const SlotCaps& c, bool keep_pa
 // whose res would create the same aligned node twice falls back to the serial walk. Returns
 // false when that fallback is needed (nothing has been modified then).
 template <int NT>
-__device__ bool add_path_parallel(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path,
+__device__ __forceinline__ bool add_path_parallel(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path,
                                   int seq_id, int m, int path0, int* cur) {
   // 1. claim empty aligned slots (key al*NB+b) for the earliest res entry: atomicMin of
   //    INT32_MIN + r (every claim is below -1, the empty mark; node ids are >= 0)
@@ -389,7 +390,7 @@ constexpr int KAHN_STK = 512;
 // int16 (offsets N + 1, targets E), in-degrees as bytes, a KAHN_STK-entry stack. Returns false
 // when the graph does not fit, an in-degree exceeds 255 or the stack would overflow.
 template <int NT>
-__device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
+__device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   const int N = S.n_nodes, E = S.n_out_e;
   if (N > 32767 || E > 32767 || 2LL * (N + 1 + E + KAHN_STK) + N > lds_bytes) return false;
   int16_t* ooff = reinterpret_cast<int16_t*>(lds);  // N + 1
@@ -497,7 +498,7 @@ __device__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
 }
 
 // Kahn order on the HBM arrays (one lane), for graphs whose adjacency does not fit in LDS.
-__device__ void kahn_global(Arena& A, WinShared& S) {
+__device__ __forceinline__ void kahn_global(Arena& A, WinShared& S) {
   const int N = S.n_nodes;
   int* deg = A.t0;
   int* stk = A.t1;
@@ -593,7 +594,7 @@ __device__ __noinline__ void tree_depth_lds(int n, const int16_t* par, int16_t* 
 }
 
 template <int NT>
-__device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
+__device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
                          int lds_bytes) {
   const int64_t tk = now();
   if (!kahn_lds<NT>(A, S, lds, lds_bytes)) {
@@ -642,13 +643,11 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
       k++;
       if (w > wmax) { wmax = w; max_pre = pr; }
     }
-    int omax = -1, max_suc = -1, far_succ = 0;
+    int omax = -1, max_suc = -1;
     for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
       const int sr = A.rank[A.oe_dst[e]];
-      if (sr - r >= RING_ROWS) far_succ = 1;
       if (A.oe_w[e] > omax) { omax = A.oe_w[e]; max_suc = sr; }
     }
-    if (far_succ) pk |= RR_FAR;
     int* rec = A.rowrec + 12LL * r;
     rec[2] = k0;
     rec[3] = k - k0;
@@ -683,7 +682,7 @@ __device__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_le
 
 // graph::topsort op=1 (src/graph.cpp:24-57): LIFO order over aligned-node groups; members
 // inherit their column head's rank. Returns the number of groups.
-__device__ int topsort1(Arena& A, WinShared& S) {
+__device__ __forceinline__ int topsort1(Arena& A, WinShared& S) {
   const int N = S.n_nodes;
   int* deg = A.t0;
   int* stk = A.t1;
@@ -720,7 +719,7 @@ __device__ int topsort1(Arena& A, WinShared& S) {
 // graph::build_consensus (src/graph.cpp:294-350): FIFO BFS from the source; each node keeps
 // the predecessor with the largest in-weight, ties to the larger accumulated score. Writes the
 // consensus bases to out (reversed trace) and returns its length, or -1 if over cap.
-__device__ int consensus(Arena& A, WinShared& S, uint8_t* out, int cap) {
+__device__ __forceinline__ int consensus(Arena& A, WinShared& S, uint8_t* out, int cap) {
   const int N = S.n_nodes;
   int* deg = A.t0;
   int* queue = A.t1;
@@ -984,7 +983,7 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
 // Returns the first argmax column (local) of the row's H, for the adaptive band; *np_in gets
 // the number of in-window predecessors.
 template <int NT, int CPL, bool QLDS, class RingT>
-__device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
+__device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
                       const RowRec& rr, int bs, int be, bool want_argmax, int* np_in,
                       int& meta_row, int& meta_bsbe) {
   constexpr int RING = RingT::kRing, WR = RingT::kWR;
@@ -1096,7 +1095,7 @@ __device__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc
 // Static band per row (src/align.cpp:1378-1394) and ragged row offsets; all lanes.
 // Returns false on arena overflow.
 template <int NT>
-__device__ bool plan_static(const Arena& A, WinShared& S, const DpScores& sc, const SlotCaps& cap,
+__device__ __forceinline__ bool plan_static(const Arena& A, WinShared& S, const DpScores& sc, const SlotCaps& cap,
                             int w) {
   const int n = S.n_rows, m = S.m;
   const int beg = S.beg_i, end = S.end_i;
@@ -1157,7 +1156,7 @@ __device__ bool plan_static(const Arena& A, WinShared& S, const DpScores& sc, co
 
 // Backtrack (src/align.cpp:1572-1736), one lane. Emits res[acj-1] for every consumed query
 // column, so the result is already in forward order.
-__device__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc) {
+__device__ __forceinline__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc) {
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
   auto Hat = [&](int r, int j) { return A.H[A.row_off[r] + j]; };
@@ -1296,7 +1295,7 @@ __device__ __forceinline__ int bt_pick(bool cand, int w, int lane) {
   return best < 0 ? -1 : 63 - (best & 63);
 }
 
-__device__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc) {
+__device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc) {
   const int lane = threadIdx.x & 63;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
@@ -1436,7 +1435,7 @@ __device__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc) 
 // a predecessor whose base is the query's is that node; otherwise the node aligned to its
 // column head with the query base, or a new node aligned to the head. All threads.
 template <int NT>
-__device__ void resolve_res(const Arena& A, const WinShared& S) {
+__device__ __forceinline__ void resolve_res(const Arena& A, const WinShared& S) {
   const int beg = S.beg_i;
   for (int x = threadIdx.x; x < S.m; x += NT) {
     const int p = A.res_from[x];
@@ -1464,7 +1463,7 @@ __device__ void resolve_res(const Arena& A, const WinShared& S) {
 
 
 // Adaptive band of row i before its DP (src/align.cpp:1433-1444); one lane.
-__device__ void adaptive_band_row(const Arena& A, WinShared& S, int i, int w) {
+__device__ __forceinline__ void adaptive_band_row(const Arena& A, WinShared& S, int i, int w) {
   const int beg = S.beg_i, end = S.end_i, m = S.m;
   const int aci = beg + i;
   const int lo = A.hlen[aci] - A.hlen[beg], hi = m - A.tlen[aci] + A.tlen[end];
@@ -1476,7 +1475,7 @@ __device__ void adaptive_band_row(const Arena& A, WinShared& S, int i, int w) {
 }
 
 // Adaptive-band update after row i (src/align.cpp:1516-1568); one lane, argmax given.
-__device__ void adaptive_update(const Arena& A, WinShared& S, int i, int max_j, int w) {
+__device__ __forceinline__ void adaptive_update(const Arena& A, WinShared& S, int i, int max_j, int w) {
   const int beg = S.beg_i, n = S.n_rows;
   const int max_acj = A.Bs[i] * Q + max_j;
   const int u = A.rank_id[beg + i];
@@ -1529,18 +1528,20 @@ __device__ void adaptive_update(const Arena& A, WinShared& S, int i, int max_j, 
 // LDS ring shape per workgroup size: RING rows of WR columns (int32 H and D).
 template <int NT>
 struct RingShape {
-  static constexpr int RING = RING_ROWS;
-  // 64/128/256: the batched tiles, sized so that the sort of a graph of up to ~3,000 /
-  // ~4,300 / ~7,000 nodes runs in LDS while leaving room for small-tile workgroups on the
-  // same CU (27 / 38 / 63 KB). 512: one window alone (minipoa's own run), wide bands.
-  static constexpr int WR = NT == 64 ? 368 : NT == 128 ? 512 : NT == 256 ? 896 : 2048;
-  // longest query held in LDS
-  static constexpr int QCAP = NT == 64 ? 4 * WR + 64 : NT <= 256 ? 4096 : 24576;
+  // 64: six rows of 344 columns, 20 KB in all, so that eight workgroups share a CU (the
+  // seventh and eighth most recent rows serve ~4% of the predecessor reads; concurrency
+  // matters more). 128/256: the larger batched tiles, sized so that the sort of a graph of up
+  // to ~5,400 / ~8,000 nodes runs in LDS while leaving room for small-tile workgroups on the
+  // same CU (39 / 63 KB). 512: one window alone (minipoa's own run), wide bands.
+  static constexpr int RING = NT == 64 ? 6 : RING_ROWS;
+  static constexpr int WR = NT == 64 ? 344 : NT == 128 ? 512 : NT == 256 ? 896 : 2048;
+  // longest query held in LDS (64: queries of up to 1,500 bases)
+  static constexpr int QCAP = NT == 64 ? 1536 : NT <= 256 ? 4096 : 24576;
 };
 
 // One poa() call on the device graph: rows [beg_i, end_i] of the current rank order.
 template <int NT, int CPL, bool QLDS, class RingT>
-__device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, bool adaptive) {
+__device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, bool adaptive) {
   constexpr int RING = RingT::kRing, WR = RingT::kWR;
   const DpScores& sc = a.scores;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
@@ -1700,7 +1701,7 @@ __device__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, 
 // (align.cpp:1362-1373); plain in A.qseq and shifted one column right in A.qsh and, when it
 // fits, R.q (entry c is the character column c consumes in the M-source, seq[c-1]).
 template <int NT, class RingT>
-__device__ void load_query(Arena& A, WinShared& S, RingT& R, const uint8_t* src, int qlen) {
+__device__ __forceinline__ void load_query(Arena& A, WinShared& S, RingT& R, const uint8_t* src, int qlen) {
   const int tid = threadIdx.x;
   const int m = qlen + 1;
   const int col_size = (m + 1 + Q - 1) / Q * Q;
@@ -1723,7 +1724,7 @@ __device__ void load_query(Arena& A, WinShared& S, RingT& R, const uint8_t* src,
 
 // First read: the source, then one insertion per base (align.cpp:1747-1753).
 template <int NT>
-__device__ void trivial_res(Arena& A, int m) {
+__device__ __forceinline__ void trivial_res(Arena& A, int m) {
   for (int j = threadIdx.x; j < m; j += NT) {
     A.res_from[j] = j == 0 ? 0 : -1;
     A.res_base[j] = j == 0 ? (uint8_t)CODE_N : A.qseq[j];
@@ -1736,7 +1737,7 @@ __device__ void trivial_res(Arena& A, int m) {
 // source, each node keeping the predecessor of largest in-weight (ties to the larger
 // accumulated score); the path back from the sink, in order, with each node's rank.
 // One lane. Returns the consensus length.
-__device__ int build_consensus_path(Arena& A, WinShared& S) {
+__device__ __forceinline__ int build_consensus_path(Arena& A, WinShared& S) {
   const int N = S.n_nodes;
   int* deg = A.t0;
   int* queue = A.t1;
@@ -1785,7 +1786,7 @@ __device__ int build_consensus_path(Arena& A, WinShared& S) {
 // op=0 (with hlen/tlen for the static or adaptive band) and, when seeding, the consensus
 // the next read is seeded against. lds / lds_bytes: LDS scratch for the sort.
 template <int NT>
-__device__ void commit_read(const KernelArgs& a, Arena& A, WinShared& S, int r, int m,
+__device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinShared& S, int r, int m,
                             int path0, bool last, bool seeding, uint8_t* lds, int lds_bytes) {
   const int tid = threadIdx.x;
   const bool keep_path = a.scores.result != 0;
@@ -1894,7 +1895,7 @@ __device__ int64_t gfa_text(const KernelArgs& a, Arena& A, WinShared& S, const W
 
 // Output text of a finished window and its result record.
 template <int NT>
-__device__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int w) {
+__device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int w) {
   const WinDesc& wd = a.win[w];
   const int tid = threadIdx.x;
   const int64_t t_out = now();
@@ -1957,7 +1958,7 @@ __device__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int 
 }
 
 template <int NT>
-__device__ void window_begin(const KernelArgs& a, Arena& A, WinShared& S, int w) {
+__device__ __forceinline__ void window_begin(const KernelArgs& a, Arena& A, WinShared& S, int w) {
   if (threadIdx.x == 0) {
     a.result[w].t_begin = now();
     S.status = MPOA_OK;
@@ -1973,7 +1974,7 @@ __device__ void window_begin(const KernelArgs& a, Arena& A, WinShared& S, int w)
 
 // The whole per-window pipeline (no seeding): every read against the whole graph.
 template <int NT, int CPL, class RingT>
-__device__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, int w) {
+__device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, int w) {
   const WinDesc& wd = a.win[w];
   const int tid = threadIdx.x;
   const bool keep_path = a.scores.result != 0;
@@ -2225,6 +2226,18 @@ bool mpoa_tile_supported(int nt, int cpl) {
   MPOA_TILES(MPOA_T)
 #undef MPOA_T
   return false;
+}
+
+int mpoa_tile_occupancy(int nt, int cpl) {
+  using namespace mpoa;
+  int blocks = 0;
+#define MPOA_O(NT_, CPL_)                                                                    \
+  if (nt == NT_ && cpl == CPL_)                                                             \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_windows<NT_, CPL_>, NT_,  \
+                                                        0) == hipSuccess ? blocks : 0;
+  MPOA_TILES(MPOA_O)
+#undef MPOA_O
+  return 0;
 }
 
 // Host-side launch (pipeline.h).
