@@ -436,7 +436,9 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
       if (z && !over) stk[sp + __popcll(mk & ((1ULL << lane) - 1))] = (int16_t)v;
       sp += __popcll(mk);
     }
-    int r = 0;
+    // Ranks leave in blocks of 64 (lane k holds the node of rank r - (r & 63) + k): two vector
+    // stores per 64 pops instead of two single-lane stores per pop.
+    int r = 0, buf_u = 0;
     int next = -1, next_o0 = 0, next_o1 = 0;
     while (!over) {
       int u, o0, o1;
@@ -450,9 +452,10 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
         o0 = ooff[u];
         o1 = ooff[u + 1];
       }
-      if (lane == 0) {
-        A.rank_id[r] = u;
-        A.rank[u] = r;
+      if (lane == (r & 63)) buf_u = u;
+      if ((r & 63) == 63) {
+        A.rank_id[r - 63 + lane] = buf_u;
+        A.rank[buf_u] = r - 63 + lane;
       }
       r++;
       const int cnt = o1 - o0;
@@ -487,6 +490,10 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
           next_o1 = __builtin_amdgcn_readlane(vo1, last);
         }
       }
+    }
+    if (lane < (r & 63)) {
+      A.rank_id[r - (r & 63) + lane] = buf_u;
+      A.rank[buf_u] = r - (r & 63) + lane;
     }
     if (lane == 0) {
       S.n_rows = r;
@@ -1285,7 +1292,8 @@ __device__ __forceinline__ int backtrack(const Arena& A, WinShared& S, const DpS
 // round trip to HBM for the cells, plus one for the predecessors' records when the row
 // changes; the "largest in-weight, first wins" choices are wave maxima of
 // (weight << 6 | 63 - k). Moves are logged as local predecessor ranks (-1: insertion) in
-// res_from and turned into res_t entries afterwards by every thread (resolve_res).
+// `log` (LDS when the query fits) and turned into res_t entries afterwards by every thread
+// (resolve_res).
 // Returns MPOA_OK, an error, or 1 when a row has more than 63 predecessors (the caller then
 // runs backtrack()). Called by wave 0 only.
 // Best candidate lane for the backtrack: largest weight, lowest lane; -1 if none.
@@ -1295,7 +1303,8 @@ __device__ __forceinline__ int bt_pick(bool cand, int w, int lane) {
   return best < 0 ? -1 : 63 - (best & 63);
 }
 
-__device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc) {
+__device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc,
+                                              int* log) {
   const int lane = threadIdx.x & 63;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
@@ -1405,7 +1414,7 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
     }
     if (kind == 0) return MPOA_E_BACKTRACK;
     if (kind != 2) {
-      if (lane == 0) A.res_from[acj - 1] = kind == 1 ? __builtin_amdgcn_readlane(p, k) : -1;
+      if (lane == 0) log[acj - 1] = kind == 1 ? __builtin_amdgcn_readlane(p, k) : -1;
       acj--;
     }
     op = nop;
@@ -1431,16 +1440,18 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
   return MPOA_OK;
 }
 
-// res_t entries from the logged moves of backtrack_wave (emit_sub of align.cpp:1603-1624):
-// a predecessor whose base is the query's is that node; otherwise the node aligned to its
-// column head with the query base, or a new node aligned to the head. All threads.
+// res_t entries from the logged moves of backtrack_wave (emit_sub of align.cpp:1603-1624; the
+// log holds a local predecessor rank per query column, -1 for an insertion): a predecessor
+// whose base is the query's is that node; otherwise the node aligned to its column head with
+// the query base, or a new node aligned to the head. All threads.
 template <int NT>
-__device__ __forceinline__ void resolve_res(const Arena& A, const WinShared& S) {
+__device__ __forceinline__ void resolve_res(const Arena& A, const WinShared& S, const int* log) {
   const int beg = S.beg_i;
   for (int x = threadIdx.x; x < S.m; x += NT) {
-    const int p = A.res_from[x];
+    const int p = log[x];
     const int qc = A.qseq[x];
     if (p < 0) {
+      A.res_from[x] = -1;
       A.res_base[x] = (uint8_t)qc;
       A.res_aligned[x] = -1;
       continue;
@@ -1679,8 +1690,11 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     S.call_width = wmax;
     S.score = A.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)];
   }
+  // The backtrack's move log goes to LDS (the ring's rows are dead now) when the query fits: a
+  // global store there would make the walk's next loads wait for it as well.
+  int* log = m <= 2 * RING * WR ? reinterpret_cast<int*>(&R) : A.res_from;  // H, D first
   if (__builtin_amdgcn_readfirstlane(tid >> 6) == 0) {  // wave 0 (a wave-uniform branch)
-    const int rc = backtrack_wave(A, S, sc);
+    const int rc = backtrack_wave(A, S, sc, log);
     if (tid == 0) {
       // A row with more than 63 predecessors: the one-lane backtrack, which writes the
       // res_t entries itself.
@@ -1689,7 +1703,7 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     }
   }
   __syncthreads();
-  if (!S.bt_serial && S.bt_rc == MPOA_OK) resolve_res<NT>(A, S);
+  if (!S.bt_serial && S.bt_rc == MPOA_OK) resolve_res<NT>(A, S, log);
   if (tid == 0) {
     if (S.bt_rc != MPOA_OK) S.status = S.bt_rc;
     S.ticks[PH_BT] += now() - t2;
