@@ -893,7 +893,7 @@ __device__ __forceinline__ void store_cols(P base, int idx, const int* v) {
 // range masks discard. Column cb's diagonal is H[cb-1], one scalar load.
 template <int CPL, class P>
 __device__ __forceinline__ void accumulate(P Hrow, P Drow, int len, int pofs, int cb, int c0,
-                                           int mlo, int mhi, int dlo, int dhi, int pbase,
+                                           int c1, int mlo, int mhi, int dlo, int dhi, int pbase,
                                            const int* qc, int match, int mismatch, int o, int e,
                                            int* h, int* d) {
   const int idx = cb - pofs;
@@ -902,6 +902,18 @@ __device__ __forceinline__ void accumulate(P Hrow, P Drow, int len, int pofs, in
   load_cols<CPL>(Hrow, safe, hv);
   load_cols<CPL>(Drow, safe, dv);
   const int hprev = Hrow[min(max(idx - 1, 0), len - 1)];
+  // A predecessor whose bands cover the row's whole pass (the common case, a wave-uniform
+  // test) needs no per-column masks.
+  const bool whole = mlo <= c0 && mhi >= c1 && dlo <= c0 && dhi >= c1;
+  if (whole) {
+#pragma unroll
+    for (int l = 0; l < CPL; l++) {
+      const int hp = l == 0 ? (cb == c0 ? NEG_INF : hprev) : hv[l - 1];
+      h[l] = max(h[l], hp + (pbase == qc[l] ? match : mismatch));
+      d[l] = max(d[l], max(dv[l] + e, hv[l] + o));
+    }
+    return;
+  }
   const int mlo_l = mlo - cb, mhi_l = mhi - cb, dlo_l = dlo - cb, dhi_l = dhi - cb;
 #pragma unroll
   for (int l = 0; l < CPL; l++) {
@@ -966,13 +978,13 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
     const int bs = bsbe & 0xffff, be = bsbe >> 16;
     const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
     const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
-    accumulate<CPL>(R.H + slot * WR, R.D + slot * WR, WR, bs * Q, cb, c0, mlo, mhi, dlo, dhi,
+    accumulate<CPL>(R.H + slot * WR, R.D + slot * WR, WR, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
                     pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
   } else {
     const int bs = A.Bs[p], be = A.Be[p], off = A.row_off[p];
     const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
     const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
-    accumulate<CPL>(A.H + off, A.D + off, (be - bs) * Q, bs * Q, cb, c0, mlo, mhi, dlo, dhi,
+    accumulate<CPL>(A.H + off, A.D + off, (be - bs) * Q, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
                     pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
     drain_vmem();
   }
