@@ -87,6 +87,19 @@ def test_batch_matches_oracle(ctx):
     assert got == want
 
 
+@pytest.mark.parametrize("result", [1, 2])
+def test_batch_msa_gfa_matches_oracle(ctx, result):
+    """Row-column MSA and GFA through the batched launches (the node paths of every read)."""
+    b = synth.config3(seed=23, n_windows=48)
+    p = api.Params(result=result)
+    _say("batch 48 windows, result", result)
+    bat = api.Batch(ctx, p, b.win_off, b.seq, b.seq_off)
+    bat.execute()
+    got = bat.results().texts
+    bat.free()
+    assert got == oracle_lib.batch(p, b.win_off, b.seq, b.seq_off)
+
+
 @pytest.mark.parametrize("result", [0, 1])
 def test_seeded_windows(ctx, result):
     """-S: host seeding, one workgroup per sub-window, adaptive band (align.cpp:1763-1827)."""
