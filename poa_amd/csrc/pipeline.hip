@@ -382,21 +382,21 @@ __device__ __forceinline__ int block_excl_sum(int v, WinShared& S, int* total) {
   return before + incl - v;
 }
 
-// Stack capacity of the LDS walk: the stack holds the released nodes not yet popped, a few
-// dozen in an alignment graph; a walk that would exceed it starts over on the HBM arrays.
+// Stack capacity of the wave walks: the stack holds the released nodes not yet popped, fewer
+// than ten in these alignment graphs; a walk that would exceed it starts over with one lane.
 constexpr int KAHN_STK = 512;
 
-// Kahn order into A.rank_id[0..N) using LDS (lds_bytes available at lds): out-edge CSR as
-// int16 (offsets N + 1, targets E), in-degrees as bytes, a KAHN_STK-entry stack. Returns false
-// when the graph does not fit, an in-degree exceeds 255 or the stack would overflow.
-template <int NT>
-__device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
-  const int N = S.n_nodes, E = S.n_out_e;
-  if (N > 32767 || E > 32767 || 2LL * (N + 1 + E + KAHN_STK) + N > lds_bytes) return false;
-  int16_t* ooff = reinterpret_cast<int16_t*>(lds);  // N + 1
-  int16_t* odst = ooff + N + 1;
-  int16_t* stk = odst + E;
-  uint8_t* deg = reinterpret_cast<uint8_t*>(stk + KAHN_STK);
+// The LIFO Kahn walk of graph::topsort op=0 (src/graph.cpp:59-123) into A.rank_id / A.rank,
+// over an out-edge CSR (ooff: N + 1 offsets, odst: targets in out-list order), in-degrees deg
+// and a stack of KAHN_STK entries. Two storage forms share it: the LDS form (int16 CSR, byte
+// degrees) and, for graphs too big for the tile's LDS, the HBM form (int32 CSR in the slot's
+// scratch arrays, degrees decremented with atomics so that every access to a counter is ordered
+// at L2; the stack stays in LDS). Returns false when a degree exceeds deg_max or the stack would
+// overflow (the caller then walks with one lane).
+template <int NT, bool HBM, class Off, class Deg, class Stk>
+__device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off* odst, Deg* deg,
+                                          Stk* stk, int deg_max) {
+  const int N = S.n_nodes;
   // out-degree prefix sums (node id order)
   int base = 0, wide = 0;
   for (int v0 = 0; v0 < N; v0 += NT) {
@@ -406,17 +406,17 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
     const int ex = block_excl_sum<NT>(oc, S, &tot);
     if (v < N) {
       const int ic = A.in_cnt[v];
-      wide |= ic > 255;
-      ooff[v] = (int16_t)(base + ex);
-      deg[v] = (uint8_t)ic;
+      wide |= ic > deg_max;
+      ooff[v] = (Off)(base + ex);
+      deg[v] = (Deg)ic;
     }
     base += tot;
   }
-  if (threadIdx.x == 0) ooff[N] = (int16_t)base;
+  if (threadIdx.x == 0) ooff[N] = (Off)base;
   if (__syncthreads_or(wide)) return false;
   for (int v = threadIdx.x; v < N; v += NT) {
     int k = ooff[v];
-    for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) odst[k++] = (int16_t)A.oe_dst[e];
+    for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) odst[k++] = (Off)A.oe_dst[e];
   }
   __syncthreads();
   // The walk, by wave 0. Zero in-degree nodes go on the stack in ascending id order; each
@@ -433,7 +433,7 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
       const bool z = v < N && deg[v] == 0;
       const uint64_t mk = __ballot(z);
       over = sp + __popcll(mk) > KAHN_STK;
-      if (z && !over) stk[sp + __popcll(mk & ((1ULL << lane) - 1))] = (int16_t)v;
+      if (z && !over) stk[sp + __popcll(mk & ((1ULL << lane) - 1))] = (Stk)v;
       sp += __popcll(mk);
     }
     // Ranks leave in blocks of 64 (lane k holds the node of rank r - (r & 63) + k): two vector
@@ -466,8 +466,13 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
         bool z = false;
         if (k < cnt) {
           v = odst[o0 + k];
-          const int d = deg[v] - 1;
-          deg[v] = (uint8_t)d;
+          int d;
+          if constexpr (HBM) {
+            d = atomicSub(&deg[v], 1) - 1;
+          } else {
+            d = deg[v] - 1;
+            deg[v] = (Deg)d;
+          }
           z = d == 0;
           vo0 = ooff[v];
           vo1 = ooff[v + 1];
@@ -479,11 +484,11 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
         }
         if (mk) {
           // an earlier pending next goes under the new pushes
-          if (next >= 0 && lane == 0) stk[sp] = (int16_t)next;
+          if (next >= 0 && lane == 0) stk[sp] = (Stk)next;
           if (next >= 0) sp++;
           const int last = 63 - __clzll(mk);
           const uint64_t rest = mk & ~(1ULL << last);
-          if (z && lane != last) stk[sp + __popcll(rest & ((1ULL << lane) - 1))] = (int16_t)v;
+          if (z && lane != last) stk[sp + __popcll(rest & ((1ULL << lane) - 1))] = (Stk)v;
           sp += __popcll(rest);
           next = __builtin_amdgcn_readlane(v, last);
           next_o0 = __builtin_amdgcn_readlane(vo0, last);
@@ -504,7 +509,31 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
   return !S.kahn_over;
 }
 
-// Kahn order on the HBM arrays (one lane), for graphs whose adjacency does not fit in LDS.
+// The walk with the graph in LDS (lds_bytes available at lds): int16 CSR, byte in-degrees, a
+// KAHN_STK-entry int16 stack. False when it does not fit, a degree exceeds 255 or the stack
+// would overflow.
+template <int NT>
+__device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
+  const int N = S.n_nodes, E = S.n_out_e;
+  if (N > 32767 || E > 32767 || 2LL * (N + 1 + E + KAHN_STK) + N > lds_bytes) return false;
+  int16_t* ooff = reinterpret_cast<int16_t*>(lds);  // N + 1
+  int16_t* odst = ooff + N + 1;
+  int16_t* stk = odst + E;
+  uint8_t* deg = reinterpret_cast<uint8_t*>(stk + KAHN_STK);
+  return kahn_walk<NT, false>(A, S, ooff, odst, deg, stk, 255);
+}
+
+// The walk with the graph in HBM: the CSR in pred_off (N + 1) and pred_rank (E), the degrees in
+// t0, all three free until the DP plan is filled after the sort; the stack in LDS.
+template <int NT>
+__device__ __forceinline__ bool kahn_hbm(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
+  if ((int)sizeof(int) * KAHN_STK > lds_bytes) return false;
+  return kahn_walk<NT, true>(A, S, A.pred_off, A.pred_rank, A.t0, reinterpret_cast<int*>(lds),
+                             0x7fffffff);
+}
+
+// Kahn order on the HBM lists (one lane), for a graph whose wave walk would pass KAHN_STK
+// stack entries.
 __device__ __forceinline__ void kahn_global(Arena& A, WinShared& S) {
   const int N = S.n_nodes;
   int* deg = A.t0;
@@ -604,7 +633,7 @@ template <int NT>
 __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
                          int lds_bytes) {
   const int64_t tk = now();
-  if (!kahn_lds<NT>(A, S, lds, lds_bytes)) {
+  if (!kahn_lds<NT>(A, S, lds, lds_bytes) && !kahn_hbm<NT>(A, S, lds, lds_bytes)) {
     if (threadIdx.x == 0) kahn_global(A, S);
     __syncthreads();
   }
