@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "tuned_gfx950.json")
 
 SMALL_TILES = [(64, 2), (64, 4), (64, 8)]
-MOVE_TOP = [0, 64, 128, 192, 256, 384]
+MOVE_TOP = [0, 32, 64, 96, 128, 192, 256]
 
 
 def load() -> dict | None:

@@ -856,8 +856,11 @@ __device__ __forceinline__ int wave_incl_max(int v) {
 }
 
 // Exclusive prefix max of v over the workgroup's threads, and the maximum over all of them.
+// The wave partials alternate between two halves of S.scan (ph, flipped by the caller on
+// every call): a call's reads of one half are then ordered before the next write to it by the
+// barrier of the call in between, so one barrier per call suffices.
 template <int NT>
-__device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total) {
+__device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total, int ph) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int incl = wave_incl_max(v);
   int excl = __builtin_amdgcn_update_dpp(SCAN_ID, incl, 0x138, 0xf, 0xf, false);  // wave_shr:1
@@ -867,16 +870,16 @@ __device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total) {
     *total = wtot;
     return excl;
   }
-  if (lane == 0) S.scan[wave] = wtot;
+  int* part = S.scan + 8 * ph;
+  if (lane == 0) part[wave] = wtot;
   lds_barrier<NT>();
   int before = SCAN_ID, all = SCAN_ID;
 #pragma unroll
   for (int w2 = 0; w2 < NT / 64; w2++) {
-    const int t = S.scan[w2];
+    const int t = part[w2];
     if (w2 < wave) before = max(before, t);
     all = max(all, t);
   }
-  lds_barrier<NT>();
   *total = all;
   return max(excl, before);
 }
@@ -1033,7 +1036,7 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
 template <int NT, int CPL, bool QLDS, class RingT>
 __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
                       const RowRec& rr, int bs, int be, bool want_argmax, int* np_in,
-                      int& meta_row, int& meta_bsbe) {
+                      int& meta_row, int& meta_bsbe, int& scan_ph) {
   constexpr int RING = RingT::kRing, WR = RingT::kWR;
   const int tid = threadIdx.x;
   const int c0 = bs * Q, W = (be - bs - 1) * Q, c1 = c0 + W;
@@ -1088,7 +1091,8 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
       lane_max = max(lane_max, t[l]);
     }
     int total;
-    int run = max(block_excl_max<NT>(lane_max, S, &total), carry);
+    int run = max(block_excl_max<NT>(lane_max, S, &total, scan_ph), carry);
+    scan_ph ^= 1;
     if (cb < c1) {  // W and cb - c0 are multiples of CPL: a thread's columns are all in or out
       int hv[CPL], iv[CPL];
 #pragma unroll
@@ -1627,6 +1631,7 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
   }
   // Ring directory: lane k holds the row index (-1: none) and band of ring slot k.
   int meta_row = -1, meta_bsbe = 0;
+  int scan_ph = 0;  // half of S.scan the next row scan writes
   // Row 0 (align.cpp:1415-1425): M, D = NEG_INF over blocks [Bs0, Be0) from the row start;
   // M[0][0] = 0. I of row 0 is never read; it is set to NEG_INF. Bs0 is always 0.
   {
@@ -1710,7 +1715,7 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     }
     int np_in = 0;
     const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in,
-                                         meta_row, meta_bsbe);
+                                         meta_row, meta_bsbe, scan_ph);
     lds_barrier<NT>();
     cells += W;
     ecells += (int64_t)np_in * W;
