@@ -131,6 +131,12 @@ struct WinShared {
 
 __device__ __forceinline__ int64_t now() { return (int64_t)wall_clock64(); }
 
+// Waits for this wave's outstanding HBM loads (and stores) only: s_waitcnt vmcnt(0) with the
+// export and LDS counters left at their maxima. Ends every branch that reads HBM inside a DP
+// row, so that at the branch's join no load is pending and the compiler's wait insertion has
+// no reason to wait there, on the LDS-only path, for the previous row's stores.
+__device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------------------------------
 // Graph edits (one lane). Follow graph::add_adj / node_t::add_in_adj / add_out_adj
 // (src/graph.cpp:13-20, include/node.h:23-61) and graph::add_path (graph.cpp:170-235).
@@ -350,13 +356,13 @@ __device__ __forceinline__ bool add_path_parallel(Arena& A, WinShared& S, const 
 
 // graph::topsort op=0 (src/graph.cpp:59-123), split into:
 //   1. the LIFO Kahn order (stack seeded with the zero in-degree nodes in ascending id
-//      order), one lane: in LDS when the graph's out-adjacency fits (built there in parallel),
-//      else on the HBM arrays;
+//      order) by one wavefront over an out-edge CSR built in parallel: in LDS when it fits,
+//      else in the slot's HBM scratch (kahn_walk);
 //   2. in parallel over ranks: rank arrays, the DP plan (predecessor CSR in node.in order with
 //      rank, weight and base; per-row records), and the parents of the hlen/tlen trees (the
 //      max-in-weight predecessor / max-out-weight successor, first wins);
 //   3. hlen = 1 + hlen(parent) (source 0) and tlen = 1 + tlen(child) (sink 1) as depths in
-//      those trees, by pointer jumping.
+//      those trees, by blocked list ranking (tree_depth_blocked).
 
 // Block-wide exclusive prefix sum of v (NT threads); *total gets the sum over all threads.
 template <int NT>
@@ -557,46 +563,18 @@ __device__ __forceinline__ void kahn_global(Arena& A, WinShared& S) {
   S.n_rows = r;
 }
 
-// Depth of every rank in a parent forest given as par[r] (-1 at roots), by pointer jumping:
-// depth[r] = base + number of links from r to its root. par and depth are rank-indexed HBM
-// arrays; jmp is scratch. Runs on the first wavefront only: within one wavefront memory
-// operations complete in issue order, so a lane never sees a half-updated (depth, jmp) pair.
-template <int NT>
-__device__ void tree_depth(int n, const int* par, int* depth, int* jmp, int base) {
-  if (threadIdx.x < 64) {
-    for (int r = threadIdx.x; r < n; r += 64) {
-      depth[r] = par[r] >= 0 ? 1 : 0;
-      jmp[r] = par[r];
-    }
-    for (;;) {
-      int any = 0;
-      for (int r = threadIdx.x; r < n; r += 64) {
-        const int j = jmp[r];
-        if (j >= 0) {
-          const int jj = jmp[j];
-          if (jj >= 0) {
-            depth[r] += depth[j];  // links r -> j plus j -> jj
-            jmp[r] = jj;
-            any = 1;
-          }
-        }
-      }
-      if (!__any(any)) break;
-    }
-    for (int r = threadIdx.x; r < n; r += 64) depth[r] += base;
-  }
-  __syncthreads();
-}
-
-// The same depths with the forest in LDS (int16 parents, -1 at roots; UP: parents have smaller
-// ranks). Blocks of 64 ranks in parent-first order: a lane whose parent lies in an earlier
+// Depth of every rank in a parent forest given as par[r] (-1 at roots; UP: parents have smaller
+// ranks), out[r] = base + the number of links from r to its root, by blocked list ranking.
+// Blocks of 64 ranks in parent-first order: a lane whose parent lies in an earlier
 // block reads that parent's depth; links to parents inside the block are resolved by pointer
-// jumping across the lanes (Wyllie's list ranking: at most six steps for 63 links). dep is the
-// LDS copy of the depths the later blocks read; out gets base + depth. Wave 0 only: within one
-// wavefront LDS operations complete in issue order, so the next block sees every store. Not
-// inlined: inside the window kernel it would push the register allocation into scratch.
-template <bool UP>
-__device__ __noinline__ void tree_depth_lds(int n, const int16_t* par, int16_t* dep, int* out, int base) {
+// jumping across the lanes (Wyllie's list ranking: at most six steps for 63 links). dep holds
+// the depths the later blocks read; out gets base + depth. Wave 0 only. In LDS (int16 arrays)
+// operations of one wavefront complete in issue order, so the next block sees every store; in
+// HBM (int32 arrays) each block's stores are waited for and the depths are read with
+// device-coherent loads. Not inlined: inside the window kernel it would push the register
+// allocation into scratch.
+template <bool UP, bool HBM, class T>
+__device__ __noinline__ void tree_depth_blocked(int n, const T* par, T* dep, int* out, int base) {
   const int lane = threadIdx.x & 63;
   for (int b = 0; b < n; b += 64) {
     const int hi = n - 1 - b;  // highest rank of the block when parents have larger ranks
@@ -609,6 +587,8 @@ __device__ __noinline__ void tree_depth_lds(int n, const int16_t* par, int16_t* 
         d = 1;
         link = UP ? p - b : hi - p;
         done = 0;
+      } else if constexpr (HBM) {
+        d = __hip_atomic_load(dep + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
       } else {
         d = dep[p] + 1;
       }
@@ -623,9 +603,10 @@ __device__ __noinline__ void tree_depth_lds(int n, const int16_t* par, int16_t* 
       }
     }
     if (act) {
-      dep[r] = (int16_t)d;
+      dep[r] = (T)d;
       out[r] = base + d;
     }
+    if constexpr (HBM) drain_vmem();
   }
 }
 
@@ -704,13 +685,16 @@ __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps&
     const int64_t td = now();
     if (lds_len) {
       if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
-        tree_depth_lds<true>(n, lpar, ldep, A.hlen, 0);
-        tree_depth_lds<false>(n, lchd, ldep, A.tlen, 1);
+        tree_depth_blocked<true, false>(n, lpar, ldep, A.hlen, 0);
+        tree_depth_blocked<false, false>(n, lchd, ldep, A.tlen, 1);
       }
       __syncthreads();
     } else {
-      tree_depth<NT>(n, hpar, A.hlen, A.t2, 0);
-      tree_depth<NT>(n, tchd, A.tlen, A.t2, 1);
+      if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
+        tree_depth_blocked<true, true>(n, hpar, A.t2, A.hlen, 0);
+        tree_depth_blocked<false, true>(n, tchd, A.t2, A.tlen, 1);
+      }
+      __syncthreads();
     }
     if (threadIdx.x == 0) S.ticks[PH_DEPTH] += now() - td;
   }
@@ -982,11 +966,6 @@ __device__ __forceinline__ void load_q(P qs, int cb, int cap, int* qc) {
   }
 }
 
-// Waits for this wave's outstanding HBM loads (and stores) only: s_waitcnt vmcnt(0) with the
-// export and LDS counters left at their maxima. Ends every branch that reads HBM inside a DP
-// row, so that at the branch's join no load is pending and the compiler's wait insertion has
-// no reason to wait there, on the LDS-only path, for the previous row's stores.
-__device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // One predecessor's M/D-source contribution to this thread's columns (the band conditions of
 // src/align.cpp:1467 and :1484 as contiguous column ranges), from the LDS ring or from HBM.
