@@ -736,20 +736,25 @@ __device__ __forceinline__ int topsort1(Arena& A, WinShared& S) {
   return r;
 }
 
+// State of the heaviest-bundle walks (consensus, build_consensus_path), set by all threads:
+// in-degrees in t0, scores in t2, predecessors in pre. The caller synchronises before the walk.
+template <int NT>
+__device__ __forceinline__ void heaviest_init(Arena& A, const WinShared& S, int* pre) {
+  for (int v = threadIdx.x; v < S.n_nodes; v += NT) {
+    A.t0[v] = A.in_cnt[v];
+    A.t2[v] = 0;
+    pre[v] = 0;
+  }
+}
+
 // graph::build_consensus (src/graph.cpp:294-350): FIFO BFS from the source; each node keeps
 // the predecessor with the largest in-weight, ties to the larger accumulated score. Writes the
 // consensus bases to out (reversed trace) and returns its length, or -1 if over cap.
 __device__ __forceinline__ int consensus(Arena& A, WinShared& S, uint8_t* out, int cap) {
-  const int N = S.n_nodes;
-  int* deg = A.t0;
+  int* deg = A.t0;  // in-degrees, scores and predecessors set by heaviest_init
   int* queue = A.t1;
   int* score = A.t2;
   int* pre = A.hlen;
-  for (int v = 0; v < N; v++) {
-    deg[v] = A.in_cnt[v];
-    score[v] = 0;
-    pre[v] = 0;
-  }
   int qh = 0, qt = 0;
   queue[qt++] = 0;
   while (qh < qt) {
@@ -1777,16 +1782,10 @@ __device__ __forceinline__ void trivial_res(Arena& A, int m) {
 // accumulated score); the path back from the sink, in order, with each node's rank.
 // One lane. Returns the consensus length.
 __device__ __forceinline__ int build_consensus_path(Arena& A, WinShared& S) {
-  const int N = S.n_nodes;
-  int* deg = A.t0;
+  int* deg = A.t0;  // in-degrees, scores and predecessors set by heaviest_init
   int* queue = A.t1;
   int* score = A.t2;
   int* pre = A.cons_pre;
-  for (int v = 0; v < N; v++) {
-    deg[v] = A.in_cnt[v];
-    score[v] = 0;
-    pre[v] = 0;
-  }
   int qh = 0, qt = 0;
   queue[qt++] = 0;
   while (qh < qt) {
@@ -1847,10 +1846,16 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
   if (S.status != MPOA_OK) return;
   const int64_t tb = now();
   if (last) {
-    if (tid == 0) S.msa_cols = topsort1(A, S);
+    // the aligned-group order of the last read (op=1) only places the MSA columns and the GFA
+    // records; the consensus output does not read it
+    if (tid == 0 && a.scores.result != 0) S.msa_cols = topsort1(A, S);
   } else {
     topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes);
-    if (seeding && tid == 0) S.cons_len = build_consensus_path(A, S);
+    if (seeding) {
+      heaviest_init<NT>(A, S, A.cons_pre);
+      __syncthreads();
+      if (tid == 0) S.cons_len = build_consensus_path(A, S);
+    }
   }
   if (tid == 0) S.ticks[PH_SORT] += now() - tb;
   __syncthreads();
@@ -1940,6 +1945,8 @@ __device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, Wi
   const int64_t t_out = now();
   if (S.status == MPOA_OK) {
     if (a.scores.result == 0) {
+      heaviest_init<NT>(A, S, A.hlen);
+      __syncthreads();
       if (tid == 0) {
         const int len = consensus(A, S, a.out + wd.out0, wd.out_cap);
         if (len < 0) S.status = MPOA_E_OVERFLOW;
