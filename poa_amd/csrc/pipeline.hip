@@ -645,25 +645,31 @@ __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps&
     const int k0 = A.pred_off[r];
     const int ind10 = A.ind[u] / 10;
     int k = k0, wmax = -1, max_pre = -1, pr3[3] = {0, 0, 0}, w3[3] = {0, 0, 0}, pk = 0;
-    for (int e = A.in_head[u]; e != -1; e = A.ie_next[e]) {
-      const int src = A.ie_src[e], w = A.ie_w[e], pr = A.rank[src];
-      const int b = A.base[src];
-      A.pred_rank[k] = pr;
-      A.pred_w[k] = w;
-      A.pred_base[k] = (uint8_t)(b | (w >= ind10 ? 0x80 : 0));
-      if (k - k0 < 3) {
-        pr3[k - k0] = pr;
-        w3[k - k0] = w;
-        pk |= b << (3 * (k - k0));
-        if (w >= ind10) pk |= 1 << (9 + (k - k0));
-      }
-      k++;
-      if (w > wmax) { wmax = w; max_pre = pr; }
-    }
     int omax = -1, max_suc = -1;
-    for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
-      const int sr = A.rank[A.oe_dst[e]];
-      if (A.oe_w[e] > omax) { omax = A.oe_w[e]; max_suc = sr; }
+    // the in-list and the out-list are walked in one loop, so that their loads overlap
+    int ei = A.in_head[u], eo = A.out_head[u];
+    while (ei != -1 || eo != -1) {
+      if (ei != -1) {
+        const int src = A.ie_src[ei], w = A.ie_w[ei], pr = A.rank[src];
+        const int b = A.base[src];
+        ei = A.ie_next[ei];
+        A.pred_rank[k] = pr;
+        A.pred_w[k] = w;
+        A.pred_base[k] = (uint8_t)(b | (w >= ind10 ? 0x80 : 0));
+        if (k - k0 < 3) {
+          pr3[k - k0] = pr;
+          w3[k - k0] = w;
+          pk |= b << (3 * (k - k0));
+          if (w >= ind10) pk |= 1 << (9 + (k - k0));
+        }
+        k++;
+        if (w > wmax) { wmax = w; max_pre = pr; }
+      }
+      if (eo != -1) {
+        const int sr = A.rank[A.oe_dst[eo]], ow = A.oe_w[eo];
+        eo = A.oe_next[eo];
+        if (ow > omax) { omax = ow; max_suc = sr; }
+      }
     }
     int* rec = A.rowrec + 12LL * r;
     rec[2] = k0;
