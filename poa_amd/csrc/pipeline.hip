@@ -797,16 +797,15 @@ __device__ __forceinline__ int consensus(Arena& A, WinShared& S, uint8_t* out, i
 
 // Per-row DP record (12 int32, one per rank), written by the topological sort (predecessors)
 // and the band plan (band, storage offsets); read once per row.
-//   [0] row_off: offset of the row in the DP matrices (traceback words in traceback mode)
+//   [0] row_off: offset of the row in the DP matrices
 //   [1] Bs | Be << 16      [2] k0: first predecessor in the predecessor CSR   [3] n_pred
 //   [4..6] ranks of the first three predecessors
 //   [7] bits 0-8: their bases (3 bits each); bits 9-11: their in-weight >= ind/10 (the
-//       backtrack's match rule, src/align.cpp:1609); bit 12: the row is read back from HBM
-//       by a successor (one at least RING ranks later)
+//       backtrack's match rule, src/align.cpp:1609)
 //   [8..10] in-weights of the first three predecessors
-//   [11] hd_off: offset of the row's H and D copy in HBM, -1 when it has none
+//   [11] unused (keeps the record three 16-byte vectors)
 struct RowRec {
-  int row_off, bsbe, k0, np, p0, p1, p2, pk, w0, w1, w2, hd_off;
+  int row_off, bsbe, k0, np, p0, p1, p2, pk, w0, w1, w2, spare;
 };
 __device__ __forceinline__ RowRec rowrec_of(const int4& a, const int4& b, const int4& c) {
   return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
@@ -1184,7 +1183,6 @@ __device__ __forceinline__ bool plan_static(const Arena& A, WinShared& S, const 
       int* rec = A.rowrec + 12LL * (beg + r);
       rec[0] = off;
       rec[1] = A.Bs[r] | (A.Be[r] << 16);
-      rec[11] = off;
     }
     base += all;
   }
