@@ -30,7 +30,7 @@ struct SlotCaps {
 
 // Int32 and byte words a slot's arena takes; must match carve() in pipeline.hip.
 inline __host__ __device__ int64_t slot_ints(int N, int E, int qmax, int path) {
-  return 47LL * N + 8LL * E + 2LL * qmax + 8 + path;
+  return 46LL * N + 8LL * E + 2LL * qmax + 8 + path;
 }
 inline __host__ __device__ int64_t slot_bytes(int N, int E, int qmax) {
   return 2LL * N + E + 3LL * qmax + 128;

@@ -59,7 +59,7 @@ struct Arena {
   int* rowrec;  // RowRec per rank (12 ints)
   int *Mp, *Sl, *Pl, *Pr, *Ol, *Or, *Ow;
   int *res_from, *res_aligned;
-  int *cons_id, *cons_rank, *cons_pre;  // consensus path for -S seeding
+  int *cons_id, *cons_rank;  // consensus path for -S seeding
   uint8_t *res_base, *qseq, *qsh;
   uint8_t* cons_ch;
   int* path;
@@ -86,7 +86,7 @@ __device__ Arena carve(const KernelArgs& a, int slot) {
   A.Ol = take(N); A.Or = take(N); A.Ow = take(N);
   A.res_from = take(c.qmax + 2); A.res_aligned = take(c.qmax + 2);
   A.path = take(c.path);
-  A.cons_id = take(N); A.cons_rank = take(N); A.cons_pre = take(N);
+  A.cons_id = take(N); A.cons_rank = take(N);
   uint8_t* b = a.byte_pool + (int64_t)slot * c.bytes_per_slot;
   A.base = b; b += N;
   A.cons_ch = b; b += N;
@@ -742,7 +742,7 @@ __device__ __forceinline__ int topsort1(Arena& A, WinShared& S) {
   return r;
 }
 
-// State of the heaviest-bundle walks (consensus, build_consensus_path), set by all threads:
+// State of the heaviest-bundle walk of the consensus output, set by all threads:
 // in-degrees in t0, scores in t2, predecessors in pre. The caller synchronises before the walk.
 template <int NT>
 __device__ __forceinline__ void heaviest_init(Arena& A, const WinShared& S, int* pre) {
@@ -1781,47 +1781,92 @@ __device__ __forceinline__ void trivial_res(Arena& A, int m) {
   __syncthreads();
 }
 
-// graph::build_consensus (graph.cpp:294-350) into the consensus arrays: FIFO BFS from the
-// source, each node keeping the predecessor of largest in-weight (ties to the larger
-// accumulated score); the path back from the sink, in order, with each node's rank.
-// One lane. Returns the consensus length.
-__device__ __forceinline__ int build_consensus_path(Arena& A, WinShared& S) {
-  int* deg = A.t0;  // in-degrees, scores and predecessors set by heaviest_init
-  int* queue = A.t1;
+// graph::build_consensus (graph.cpp:294-350) for -S seeding, into the consensus arrays: every
+// node keeps the predecessor of largest in-weight, ties to the larger accumulated score, then
+// the first in its in-list; the path back from the sink, in order, with each node's rank.
+// minipoa visits the nodes in FIFO order, but a node's choice reads only its predecessors' final
+// scores, so any topological order gives the same path: here the rank order of the sort just
+// done, with the DP plan's predecessor CSR, by wave 0 in blocks of 64 ranks. A rank whose
+// predecessors inside its block are not final yet waits for the next round (at most 64 rounds a
+// block; each round settles at least the lowest rank left). Scores and choices go to t2 / t1 (by
+// rank), read back with device-coherent loads after each block's stores are waited for; the
+// path is chased by one lane into t0 and turned into the consensus arrays by all threads.
+template <int NT>
+__device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t* lds) {
+  const int n = S.n_rows;
   int* score = A.t2;
-  int* pre = A.cons_pre;
-  int qh = 0, qt = 0;
-  queue[qt++] = 0;
-  while (qh < qt) {
-    const int u = queue[qh++];
-    if (u != 0) {
-      int wmax = -1, max_pre = -1;
-      for (int e = A.in_head[u]; e != -1; e = A.ie_next[e]) {
-        const int v = A.ie_src[e], w = A.ie_w[e];
-        if (w > wmax || (w == wmax && score[v] > score[max_pre])) {
-          wmax = w;
-          max_pre = v;
+  int* pre = A.t1;
+  int* path = A.t0;
+  int* bdone = reinterpret_cast<int*>(lds);  // this block's settled ranks and their scores
+  int* bsc = bdone + 64;
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
+    const int lane = threadIdx.x & 63;
+    for (int b = 0; b < n; b += 64) {
+      const int r = b + lane;
+      const bool act = r < n;
+      const int k0 = act ? A.pred_off[r] : 0, k1 = act ? A.pred_off[r + 1] : 0;
+      int done = !act || A.rank_id[r] == 0;  // the source scores 0
+      int sc = 0, pr = -1;
+      bdone[lane] = done;
+      bsc[lane] = 0;
+      for (int round = 0; round < 64 && !__all(done); round++) {
+        if (!done) {
+          bool ready = true;
+          int wmax = -1, mp = -1, mps = 0;
+          for (int k = k0; k < k1; k++) {
+            const int p = A.pred_rank[k], w = A.pred_w[k];
+            int ps;
+            if (p < b) {
+              ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+              if (!bdone[p - b]) {
+                ready = false;
+                break;
+              }
+              ps = bsc[p - b];
+            }
+            if (w > wmax || (w == wmax && ps > mps)) {
+              wmax = w;
+              mp = p;
+              mps = ps;
+            }
+          }
+          if (ready) {
+            sc = mps + wmax;
+            pr = mp;
+            done = 1;
+          }
         }
+        bdone[lane] = done;  // every lane's reads of this round come first (one wavefront)
+        bsc[lane] = sc;
+        __builtin_amdgcn_wave_barrier();
       }
-      pre[u] = max_pre;
-      score[u] = score[max_pre] + wmax;
+      if (act) {
+        score[r] = sc;
+        pre[r] = pr;
+      }
+      drain_vmem();
     }
-    if (u == 1) break;
-    for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
-      const int v = A.oe_dst[e];
-      if (--deg[v] == 0) queue[qt++] = v;
+    if (lane == 0) {
+      int len = 0;
+      int x = __hip_atomic_load(pre + A.rank[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while (x >= 0 && A.rank_id[x] != 0) {
+        path[len++] = x;
+        x = __hip_atomic_load(pre + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      S.cons_len = len;
     }
   }
-  int len = 0;
-  for (int id = pre[1]; id != 0; id = pre[id]) len++;
-  int at = len;
-  for (int id = pre[1]; id != 0; id = pre[id]) {
-    --at;
+  __syncthreads();
+  const int len = S.cons_len;
+  for (int at = threadIdx.x; at < len; at += NT) {
+    const int x = path[len - 1 - at];
+    const int id = A.rank_id[x];
     A.cons_id[at] = id;
     A.cons_ch[at] = A.base[id];
-    A.cons_rank[at] = A.rank[id];
+    A.cons_rank[at] = x;
   }
-  return len;
+  __syncthreads();
 }
 
 // add_path then the re-sort of one read (main.cpp:118-120): op=1 after the last read, else
@@ -1855,11 +1900,7 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
     if (tid == 0 && a.scores.result != 0) S.msa_cols = topsort1(A, S);
   } else {
     topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes);
-    if (seeding) {
-      heaviest_init<NT>(A, S, A.cons_pre);
-      __syncthreads();
-      if (tid == 0) S.cons_len = build_consensus_path(A, S);
-    }
+    if (seeding) heaviest_ranked<NT>(A, S, lds);
   }
   if (tid == 0) S.ticks[PH_SORT] += now() - tb;
   __syncthreads();
