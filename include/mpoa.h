@@ -74,7 +74,10 @@ typedef struct mpoa_call_stat {
  * Reads: seq with n_reads+1 offsets. name/comment (optional, NULL) are laid out the same way
  * and only used by the MSA/GFA headers. Output text (exactly minipoa's stdout) goes to
  * out[0..out_cap); *out_len receives its size (also when MPOA_E_OUT_CAP is returned).
- * stats (optional) receives one record per DP call. */
+ * stats (optional) receives one record per DP call. A call whose text does not fit (out NULL,
+ * or out_cap too small) keeps the text and stats in the context until the next call: called
+ * again with the same arguments (a buffer of the size learned aside), it returns them without
+ * running the alignment again. */
 int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, const char* seq,
                      const int64_t* seq_off, const char* name, const int64_t* name_off,
                      const char* comment, const int64_t* comment_off, char* out,

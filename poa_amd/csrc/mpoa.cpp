@@ -38,6 +38,10 @@ struct mpoa_ctx {
   int n_cu = 0;
   size_t mem_total = 0;
   int wall_khz = 100000;          // device wall clock (the k_hold timer)
+  // mpoa_align_reads: the last call whose text did not fit its buffer, for the follow-up call
+  // with the same arguments
+  std::string held_key, held_text;
+  std::vector<mpoa_call_stat> held_calls;
 };
 
 namespace {
@@ -1361,6 +1365,47 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
                      int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
                      int32_t stats_cap, int32_t* n_stats) {
   if (!ctx || !params || n_reads < 0 || (n_reads > 0 && (!seq || !seq_off))) return MPOA_E_ARG;
+  // Every argument the text and the stats depend on, as bytes: a call with the same key as the
+  // held one (a call whose text did not fit) takes the held text instead of running again.
+  std::string key;
+  {
+    auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
+    auto put_list = [&](const char* data, const int64_t* off) {
+      const int flag = data && off ? 1 : 0;
+      put(&flag, sizeof flag);
+      if (!flag) return;
+      put(off, sizeof(int64_t) * ((size_t)n_reads + 1));
+      if (n_reads > 0 && off[n_reads] > off[0]) put(data + off[0], (size_t)(off[n_reads] - off[0]));
+    };
+    put(params, sizeof *params);
+    put(&n_reads, sizeof n_reads);
+    if (n_reads > 0) put_list(seq, seq_off);
+    put_list(name, name_off);
+    put_list(comment, comment_off);
+  }
+  auto deliver = [&](std::string& text, const std::vector<mpoa_call_stat>& calls) {
+    if (n_stats) *n_stats = (int32_t)calls.size();
+    if (stats)
+      for (int i = 0; i < (int)calls.size() && i < stats_cap; i++) stats[i] = calls[i];
+    if (out_len) *out_len = (int64_t)text.size();
+    if (!out || (int64_t)text.size() > out_cap) {
+      if (&text != &ctx->held_text) {
+        ctx->held_key.swap(key);
+        ctx->held_text.swap(text);
+        ctx->held_calls = calls;
+      }
+      return out ? MPOA_E_OUT_CAP : MPOA_OK;
+    }
+    std::memcpy(out, text.data(), text.size());
+    ctx->held_key.clear();
+    ctx->held_text.clear();
+    ctx->held_calls.clear();
+    return MPOA_OK;
+  };
+  if (!ctx->held_key.empty() && key == ctx->held_key) {
+    const std::vector<mpoa_call_stat> calls = ctx->held_calls;
+    return deliver(ctx->held_text, calls);
+  }
   int32_t win_off[2] = {0, n_reads};
   std::vector<int64_t> zero_off{0};
   const int64_t* so = n_reads > 0 ? seq_off : zero_off.data();
@@ -1460,11 +1505,7 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
     }
   }
   mpoa_batch_free(b);
-  if (out_len) *out_len = (int64_t)text.size();
-  if (!out) return MPOA_OK;
-  if ((int64_t)text.size() > out_cap) return MPOA_E_OUT_CAP;
-  std::memcpy(out, text.data(), text.size());
-  return MPOA_OK;
+  return deliver(text, calls);
 }
 
 }  // extern "C"

@@ -109,3 +109,40 @@ def test_seeded_windows(ctx, result):
     reads = synth.window(32, 5000, 6, 0.024, 0.048, 0.048, 0.7)
     _say("seeded config4-like", result)
     _check_window(ctx, reads, seeding=1, k=15, mm_w=10, result=result)
+
+
+def test_held_text_follow_up_calls(ctx):
+    """A call whose text does not fit keeps it for the follow-up call with the same arguments;
+    a call with other arguments in between runs its own alignment."""
+    import ctypes
+    ra = synth.window(41, 400, 7, 0.04, 0.03, 0.03, 0.7)
+    rb = synth.window(42, 350, 5, 0.04, 0.03, 0.03, 0.7)
+    p = api.Params(result=1)
+    want_a, calls_a = oracle_lib.window(p, ra.seq, ra.off)
+    want_b, _ = oracle_lib.window(p, rb.seq, rb.off)
+    L = api.lib()
+
+    def call(reads, cap):
+        seq = np.ascontiguousarray(reads.seq, np.uint8)
+        off = np.ascontiguousarray(reads.off, np.int64)
+        need, n_stats = ctypes.c_int64(), ctypes.c_int32()
+        buf = np.zeros(max(cap, 1), np.uint8) if cap is not None else None
+        stats = (api.CallStat * 64)()
+        rc = L.mpoa_align_reads(ctx.h, ctypes.byref(p), len(off) - 1, api._ptr(seq), api._ptr(off),
+                                None, None, None, None, None if buf is None else api._ptr(buf),
+                                0 if buf is None else cap, ctypes.byref(need), stats, 64,
+                                ctypes.byref(n_stats))
+        text = buf[:need.value].tobytes().decode() if buf is not None and rc == 0 else None
+        return rc, need.value, text, [stats[i].score for i in range(n_stats.value)]
+
+    _say("held text: sizes of A, then B in full, then A short and in full")
+    rc, need_a, _, _ = call(ra, None)
+    assert rc == 0 and need_a == len(want_a)
+    rc, need_b, text, _ = call(rb, 1 << 20)
+    assert rc == 0 and text == want_b
+    rc, need, _, _ = call(ra, need_a - 1)
+    assert rc == -4 and need == need_a  # MPOA_E_OUT_CAP
+    rc, need, text, scores = call(ra, need_a)
+    assert rc == 0 and text == want_a and scores == [c.score for c in calls_a]
+    rc, need, text, _ = call(ra, need_a)  # delivered: the next call runs again
+    assert rc == 0 and text == want_a
