@@ -1786,54 +1786,95 @@ __device__ __forceinline__ void trivial_res(Arena& A, int m) {
 // the first in its in-list; the path back from the sink, in order, with each node's rank.
 // minipoa visits the nodes in FIFO order, but a node's choice reads only its predecessors' final
 // scores, so any topological order gives the same path: here the rank order of the sort just
-// done, with the DP plan's predecessor CSR, by wave 0 in blocks of 64 ranks. A rank whose
-// predecessors inside its block are not final yet waits for the next round (at most 64 rounds a
-// block; each round settles at least the lowest rank left). Scores and choices go to t2 / t1 (by
-// rank), read back with device-coherent loads after each block's stores are waited for; the
-// path is chased by one lane into t0 and turned into the consensus arrays by all threads.
+// done, with the DP plan's predecessor CSR, by wave 0 in blocks of 64 ranks. Each lane reads its
+// rank's predecessors once: those in earlier blocks are final (scores in t2, read with
+// device-coherent loads after each block's stores are waited for) and reduce to one candidate;
+// up to HR_IN inside the block go to LDS, and the lane settles in the first round in which all
+// of those are settled (at most 64 rounds a block; each round settles at least the lowest rank
+// left). A lane with more in-block predecessors re-reads its whole in-list each round. The
+// choice, the largest (weight, score) and then the lowest in-list position, does not depend on
+// the order the candidates are met in. Choices go to t1; the path is chased by one lane into t0
+// and turned into the consensus arrays by all threads.
+constexpr int HR_IN = 8;
+
 template <int NT>
-__device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t* lds) {
+__device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   const int n = S.n_rows;
   int* score = A.t2;
   int* pre = A.t1;
   int* path = A.t0;
   int* bdone = reinterpret_cast<int*>(lds);  // this block's settled ranks and their scores
   int* bsc = bdone + 64;
+  int2* bin = reinterpret_cast<int2*>(bsc + 64);  // in-block predecessors: (offset | k << 6, w)
+  const int cap = min(HR_IN, (lds_bytes - 512) / (64 * 8));
   if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
     const int lane = threadIdx.x & 63;
+    int2* mine = bin + lane * HR_IN;
     for (int b = 0; b < n; b += 64) {
       const int r = b + lane;
       const bool act = r < n;
       const int k0 = act ? A.pred_off[r] : 0, k1 = act ? A.pred_off[r + 1] : 0;
       int done = !act || A.rank_id[r] == 0;  // the source scores 0
       int sc = 0, pr = -1;
+      int ew = -1, ep = -1, eps = 0, ek = INT_MAX, nin = 0;
+      bool reread = false;
+      if (!done) {
+        for (int k = k0; k < k1; k++) {
+          const int p = A.pred_rank[k], w = A.pred_w[k];
+          if (p < b) {
+            const int ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (w > ew || (w == ew && ps > eps)) {
+              ew = w; ep = p; eps = ps; ek = k;
+            }
+          } else if (nin < cap) {
+            mine[nin++] = make_int2((p - b) | (k - k0) << 6, w);
+          } else {
+            reread = true;
+          }
+        }
+      }
       bdone[lane] = done;
       bsc[lane] = 0;
+      __builtin_amdgcn_wave_barrier();
       for (int round = 0; round < 64 && !__all(done); round++) {
         if (!done) {
           bool ready = true;
-          int wmax = -1, mp = -1, mps = 0;
-          for (int k = k0; k < k1; k++) {
-            const int p = A.pred_rank[k], w = A.pred_w[k];
-            int ps;
-            if (p < b) {
-              ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-              if (!bdone[p - b]) {
+          int bw = ew, bp = ep, bps = eps, bk = ek;
+          if (!reread) {
+            for (int j = 0; j < nin; j++) {
+              const int2 e = mine[j];
+              const int o = e.x & 63, k = k0 + (e.x >> 6);
+              if (!bdone[o]) {
                 ready = false;
                 break;
               }
-              ps = bsc[p - b];
+              const int ps = bsc[o];
+              if (e.y > bw || (e.y == bw && (ps > bps || (ps == bps && k < bk)))) {
+                bw = e.y; bp = b + o; bps = ps; bk = k;
+              }
             }
-            if (w > wmax || (w == wmax && ps > mps)) {
-              wmax = w;
-              mp = p;
-              mps = ps;
+          } else {
+            bw = -1; bp = -1; bps = 0;
+            for (int k = k0; k < k1; k++) {
+              const int p = A.pred_rank[k], w = A.pred_w[k];
+              int ps;
+              if (p < b) {
+                ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              } else {
+                if (!bdone[p - b]) {
+                  ready = false;
+                  break;
+                }
+                ps = bsc[p - b];
+              }
+              if (w > bw || (w == bw && ps > bps)) {
+                bw = w; bp = p; bps = ps;
+              }
             }
           }
           if (ready) {
-            sc = mps + wmax;
-            pr = mp;
+            sc = bps + bw;
+            pr = bp;
             done = 1;
           }
         }
@@ -1900,7 +1941,7 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
     if (tid == 0 && a.scores.result != 0) S.msa_cols = topsort1(A, S);
   } else {
     topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes);
-    if (seeding) heaviest_ranked<NT>(A, S, lds);
+    if (seeding) heaviest_ranked<NT>(A, S, lds, lds_bytes);
   }
   if (tid == 0) S.ticks[PH_SORT] += now() - tb;
   __syncthreads();
