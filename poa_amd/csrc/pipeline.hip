@@ -394,14 +394,16 @@ constexpr int KAHN_STK = 512;
 
 // The LIFO Kahn walk of graph::topsort op=0 (src/graph.cpp:59-123) into A.rank_id / A.rank,
 // over an out-edge CSR (ooff: N + 1 offsets, odst: targets in out-list order), in-degrees deg
-// and a stack of KAHN_STK entries. Two storage forms share it: the LDS form (int16 CSR, byte
-// degrees) and, for graphs too big for the tile's LDS, the HBM form (int32 CSR in the slot's
-// scratch arrays, degrees decremented with atomics so that every access to a counter is ordered
-// at L2; the stack stays in LDS). Returns false when a degree exceeds deg_max or the stack would
-// overflow (the caller then walks with one lane).
-template <int NT, bool HBM, class Off, class Deg, class Stk>
-__device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off* odst, Deg* deg,
-                                          Stk* stk, int deg_max) {
+// and a stack of KAHN_STK entries. Three storage forms share it: the LDS form (int16 CSR, byte
+// degrees); for graphs whose CSR does not fit the tile's LDS, the form with byte degrees in LDS
+// and the CSR in HBM as records (REC: each out-edge's target with the target's own out-edge
+// range, one load per edge); and the HBM form (int32 CSR in the slot's scratch arrays, degrees
+// decremented with atomics so that every access to a counter is ordered at L2; the stack stays
+// in LDS). Returns false when a degree exceeds deg_max or the stack would overflow (the caller
+// then walks with one lane).
+template <int NT, bool HBM, bool REC, class Off, class Deg, class Stk>
+__device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off* odst, int4* orec,
+                                          Deg* deg, Stk* stk, int deg_max) {
   const int N = S.n_nodes;
   // out-degree prefix sums (node id order)
   int base = 0, wide = 0;
@@ -422,7 +424,11 @@ __device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off
   if (__syncthreads_or(wide)) return false;
   for (int v = threadIdx.x; v < N; v += NT) {
     int k = ooff[v];
-    for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) odst[k++] = (Off)A.oe_dst[e];
+    for (int e = A.out_head[v]; e != -1; e = A.oe_next[e]) {
+      const int d = A.oe_dst[e];
+      if constexpr (REC) orec[k++] = make_int4(d, ooff[d], ooff[d + 1], 0);
+      else odst[k++] = (Off)d;
+    }
   }
   __syncthreads();
   // The walk, by wave 0. Zero in-degree nodes go on the stack in ascending id order; each
@@ -471,7 +477,14 @@ __device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off
         int v = 0, vo0 = 0, vo1 = 0;
         bool z = false;
         if (k < cnt) {
-          v = odst[o0 + k];
+          if constexpr (REC) {
+            const int4 e = orec[o0 + k];
+            v = e.x;
+            vo0 = e.y;
+            vo1 = e.z;
+          } else {
+            v = odst[o0 + k];
+          }
           int d;
           if constexpr (HBM) {
             d = atomicSub(&deg[v], 1) - 1;
@@ -480,8 +493,10 @@ __device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off
             deg[v] = (Deg)d;
           }
           z = d == 0;
-          vo0 = ooff[v];
-          vo1 = ooff[v + 1];
+          if constexpr (!REC) {
+            vo0 = ooff[v];
+            vo1 = ooff[v + 1];
+          }
         }
         const uint64_t mk = __ballot(z);
         if (sp + __popcll(mk) > KAHN_STK) {
@@ -526,7 +541,23 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
   int16_t* odst = ooff + N + 1;
   int16_t* stk = odst + E;
   uint8_t* deg = reinterpret_cast<uint8_t*>(stk + KAHN_STK);
-  return kahn_walk<NT, false>(A, S, ooff, odst, deg, stk, 255);
+  return kahn_walk<NT, false, false>(A, S, ooff, odst, nullptr, deg, stk, 255);
+}
+
+// The walk with byte in-degrees and the stack in LDS, the CSR in HBM: offsets in pred_off, and
+// for each out-edge a record of its target and the target's own out-edge range in the row
+// records' space (free until the DP plan is filled), so that one load per edge gives the next
+// node and where its edges are. False when it does not fit, a degree exceeds 255 or the stack
+// would overflow.
+template <int NT>
+__device__ __forceinline__ bool kahn_mid(Arena& A, WinShared& S, const SlotCaps& c, uint8_t* lds,
+                                         int lds_bytes) {
+  const int N = S.n_nodes, E = S.n_out_e;
+  if ((int64_t)sizeof(int) * KAHN_STK + N > lds_bytes || 4LL * E > 12LL * c.nodes) return false;
+  int* stk = reinterpret_cast<int*>(lds);
+  uint8_t* deg = lds + sizeof(int) * KAHN_STK;
+  return kahn_walk<NT, false, true>(A, S, A.pred_off, A.pred_rank,
+                                    reinterpret_cast<int4*>(A.rowrec), deg, stk, 255);
 }
 
 // The walk with the graph in HBM: the CSR in pred_off (N + 1) and pred_rank (E), the degrees in
@@ -534,8 +565,8 @@ __device__ __forceinline__ bool kahn_lds(Arena& A, WinShared& S, uint8_t* lds, i
 template <int NT>
 __device__ __forceinline__ bool kahn_hbm(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   if ((int)sizeof(int) * KAHN_STK > lds_bytes) return false;
-  return kahn_walk<NT, true>(A, S, A.pred_off, A.pred_rank, A.t0, reinterpret_cast<int*>(lds),
-                             0x7fffffff);
+  return kahn_walk<NT, true, false>(A, S, A.pred_off, A.pred_rank, nullptr, A.t0,
+                                    reinterpret_cast<int*>(lds), 0x7fffffff);
 }
 
 // Kahn order on the HBM lists (one lane), for a graph whose wave walk would pass KAHN_STK
@@ -614,7 +645,8 @@ template <int NT>
 __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
                          int lds_bytes) {
   const int64_t tk = now();
-  if (!kahn_lds<NT>(A, S, lds, lds_bytes) && !kahn_hbm<NT>(A, S, lds, lds_bytes)) {
+  if (!kahn_lds<NT>(A, S, lds, lds_bytes) && !kahn_mid<NT>(A, S, c, lds, lds_bytes) &&
+      !kahn_hbm<NT>(A, S, lds, lds_bytes)) {
     if (threadIdx.x == 0) kahn_global(A, S);
     __syncthreads();
   }
@@ -1786,15 +1818,17 @@ __device__ __forceinline__ void trivial_res(Arena& A, int m) {
 // the first in its in-list; the path back from the sink, in order, with each node's rank.
 // minipoa visits the nodes in FIFO order, but a node's choice reads only its predecessors' final
 // scores, so any topological order gives the same path: here the rank order of the sort just
-// done, with the DP plan's predecessor CSR, by wave 0 in blocks of 64 ranks. Each lane reads its
-// rank's predecessors once: those in earlier blocks are final (scores in t2, read with
-// device-coherent loads after each block's stores are waited for) and reduce to one candidate;
-// up to HR_IN inside the block go to LDS, and the lane settles in the first round in which all
-// of those are settled (at most 64 rounds a block; each round settles at least the lowest rank
-// left). A lane with more in-block predecessors re-reads its whole in-list each round. The
-// choice, the largest (weight, score) and then the lowest in-list position, does not depend on
-// the order the candidates are met in. Choices go to t1; the path is chased by one lane into t0
-// and turned into the consensus arrays by all threads.
+// done, with the DP plan's predecessor CSR, by wave 0 in blocks of 64 ranks. The choice, the
+// largest (weight, score) and then the lowest in-list position, does not depend on the order
+// the candidates are met in, so each lane first reduces its rank's predecessors in earlier
+// blocks (final scores in t2, read with device-coherent loads after each block's stores are
+// waited for) to one candidate and keeps those inside the block (up to HR_IN: two in registers,
+// the rest in LDS). Ranks with none inside settle at once; the others settle one at a time in
+// ascending rank order, with the block's scores in one register across the lanes (readlane /
+// writelane). A block holding a rank with more in-block predecessors settles in rounds instead,
+// each lane re-reading its in-list until all of its in-block predecessors have settled. Choices
+// go to t1; the path is chased by one lane into t0 and turned into the consensus arrays by all
+// threads.
 constexpr int HR_IN = 8;
 
 template <int NT>
@@ -1803,20 +1837,21 @@ __device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t*
   int* score = A.t2;
   int* pre = A.t1;
   int* path = A.t0;
-  int* bdone = reinterpret_cast<int*>(lds);  // this block's settled ranks and their scores
-  int* bsc = bdone + 64;
-  int2* bin = reinterpret_cast<int2*>(bsc + 64);  // in-block predecessors: (offset | k << 6, w)
-  const int cap = min(HR_IN, (lds_bytes - 512) / (64 * 8));
+  int* bval = reinterpret_cast<int*>(lds);  // rounds: this block's scores, INT_MIN until settled
+  int2* bin = reinterpret_cast<int2*>(bval + 64);  // in-block predecessors past the first two
+  const int cap = 2 + max(0, min(HR_IN - 2, (lds_bytes - 256) / (64 * 8)));
+  const int xcap = cap - 2;
   if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
     const int lane = threadIdx.x & 63;
-    int2* mine = bin + lane * HR_IN;
     for (int b = 0; b < n; b += 64) {
       const int r = b + lane;
       const bool act = r < n;
       const int k0 = act ? A.pred_off[r] : 0, k1 = act ? A.pred_off[r + 1] : 0;
       int done = !act || A.rank_id[r] == 0;  // the source scores 0
-      int sc = 0, pr = -1;
+      // the best earlier-block predecessor (in-list position relative to k0); in-block ones as
+      // (offset | in-list position << 6, w)
       int ew = -1, ep = -1, eps = 0, ek = INT_MAX, nin = 0;
+      int2 e0 = make_int2(0, 0), e1 = make_int2(0, 0);
       bool reread = false;
       if (!done) {
         for (int k = k0; k < k1; k++) {
@@ -1824,63 +1859,89 @@ __device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t*
           if (p < b) {
             const int ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (w > ew || (w == ew && ps > eps)) {
-              ew = w; ep = p; eps = ps; ek = k;
+              ew = w; ep = p; eps = ps; ek = k - k0;
             }
           } else if (nin < cap) {
-            mine[nin++] = make_int2((p - b) | (k - k0) << 6, w);
+            const int2 e = make_int2((p - b) | (k - k0) << 6, w);
+            if (nin == 0) e0 = e;
+            else if (nin == 1) e1 = e;
+            else bin[lane * xcap + nin - 2] = e;
+            nin++;
           } else {
             reread = true;
           }
         }
       }
-      bdone[lane] = done;
-      bsc[lane] = 0;
-      __builtin_amdgcn_wave_barrier();
-      for (int round = 0; round < 64 && !__all(done); round++) {
-        if (!done) {
-          bool ready = true;
-          int bw = ew, bp = ep, bps = eps, bk = ek;
-          if (!reread) {
-            for (int j = 0; j < nin; j++) {
-              const int2 e = mine[j];
-              const int o = e.x & 63, k = k0 + (e.x >> 6);
-              if (!bdone[o]) {
-                ready = false;
-                break;
-              }
-              const int ps = bsc[o];
-              if (e.y > bw || (e.y == bw && (ps > bps || (ps == bps && k < bk)))) {
-                bw = e.y; bp = b + o; bps = ps; bk = k;
-              }
+      int sc = 0, pr = -1;
+      if (!done && nin == 0) {
+        sc = eps + ew;
+        pr = ep;
+        done = 1;
+      }
+      if (!__any(reread)) {
+        uint64_t todo = __ballot(!done);
+        while (todo) {
+          const int j = __builtin_ctzll(todo);
+          todo &= todo - 1;
+          int bw = __builtin_amdgcn_readlane(ew, j), bp = __builtin_amdgcn_readlane(ep, j);
+          int bps = __builtin_amdgcn_readlane(eps, j), bk = __builtin_amdgcn_readlane(ek, j);
+          const int cnt = __builtin_amdgcn_readlane(nin, j);
+          for (int c = 0; c < cnt; c++) {
+            int ex, ey;
+            if (c == 0) {
+              ex = __builtin_amdgcn_readlane(e0.x, j);
+              ey = __builtin_amdgcn_readlane(e0.y, j);
+            } else if (c == 1) {
+              ex = __builtin_amdgcn_readlane(e1.x, j);
+              ey = __builtin_amdgcn_readlane(e1.y, j);
+            } else {
+              const int2 e = bin[j * xcap + c - 2];
+              ex = __builtin_amdgcn_readfirstlane(e.x);
+              ey = __builtin_amdgcn_readfirstlane(e.y);
             }
-          } else {
-            bw = -1; bp = -1; bps = 0;
+            const int o = ex & 63, k = ex >> 6;
+            const int ps = __builtin_amdgcn_readlane(sc, o);
+            if (ey > bw || (ey == bw && (ps > bps || (ps == bps && k < bk)))) {
+              bw = ey; bp = b + o; bps = ps; bk = k;
+            }
+          }
+          if (lane == j) {
+            sc = bps + bw;
+            pr = bp;
+          }
+        }
+      } else {
+        bval[lane] = done ? sc : INT_MIN;
+        __builtin_amdgcn_wave_barrier();
+        for (int round = 0; round < 64 && !__all(done); round++) {
+          if (!done) {
+            bool ready = true;
+            int bw = -1, bp = -1, bps = 0;
             for (int k = k0; k < k1; k++) {
               const int p = A.pred_rank[k], w = A.pred_w[k];
               int ps;
               if (p < b) {
                 ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               } else {
-                if (!bdone[p - b]) {
+                ps = bval[p - b];
+                if (ps == INT_MIN) {
                   ready = false;
                   break;
                 }
-                ps = bsc[p - b];
               }
               if (w > bw || (w == bw && ps > bps)) {
                 bw = w; bp = p; bps = ps;
               }
             }
+            if (ready) {
+              sc = bps + bw;
+              pr = bp;
+              done = 1;
+            }
           }
-          if (ready) {
-            sc = bps + bw;
-            pr = bp;
-            done = 1;
-          }
+          bval[lane] = done ? sc : INT_MIN;  // every lane's reads of this round come first
+          __builtin_amdgcn_wave_barrier();
         }
-        bdone[lane] = done;  // every lane's reads of this round come first (one wavefront)
-        bsc[lane] = sc;
-        __builtin_amdgcn_wave_barrier();
       }
       if (act) {
         score[r] = sc;
@@ -2255,7 +2316,7 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
 template <int NT>
 __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
   __shared__ WinShared S;
-  __shared__ int4 lds4[(96 * 1024) / 16];
+  __shared__ int4 lds4[(156 * 1024) / 16];
   const int tid = threadIdx.x;
   const KernelArgs& a = s.graph;
   Arena A = carve(a, 0);
