@@ -1827,8 +1827,7 @@ __device__ __forceinline__ void trivial_res(Arena& A, int m) {
 // ascending rank order, with the block's scores in one register across the lanes (readlane /
 // writelane). A block holding a rank with more in-block predecessors settles in rounds instead,
 // each lane re-reading its in-list until all of its in-block predecessors have settled. Choices
-// go to t1; the path is chased by one lane into t0 and turned into the consensus arrays by all
-// threads.
+// go to t1; the path goes to t0 and is turned into the consensus arrays by all threads.
 constexpr int HR_IN = 8;
 
 template <int NT>
@@ -1854,15 +1853,36 @@ __device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t*
       int2 e0 = make_int2(0, 0), e1 = make_int2(0, 0);
       bool reread = false;
       if (!done) {
+        // the first four predecessors' loads are issued together, then their scores'
+        constexpr int PF = 4;
+        int pp[PF], pw[PF], ps4[PF];
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+          pp[i] = k0 + i < k1 ? A.pred_rank[k0 + i] : INT_MAX;
+          pw[i] = k0 + i < k1 ? A.pred_w[k0 + i] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < PF; i++)
+          ps4[i] = pp[i] < b ? __hip_atomic_load(score + pp[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
         for (int k = k0; k < k1; k++) {
-          const int p = A.pred_rank[k], w = A.pred_w[k];
+          const int i = k - k0;
+          int p, w, ps = 0;
+          if (i < PF) {
+            p = pp[0]; w = pw[0]; ps = ps4[0];
+#pragma unroll
+            for (int t = 1; t < PF; t++)
+              if (i == t) { p = pp[t]; w = pw[t]; ps = ps4[t]; }
+          } else {
+            p = A.pred_rank[k];
+            w = A.pred_w[k];
+            if (p < b) ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
           if (p < b) {
-            const int ps = __hip_atomic_load(score + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (w > ew || (w == ew && ps > eps)) {
-              ew = w; ep = p; eps = ps; ek = k - k0;
+              ew = w; ep = p; eps = ps; ek = i;
             }
           } else if (nin < cap) {
-            const int2 e = make_int2((p - b) | (k - k0) << 6, w);
+            const int2 e = make_int2((p - b) | i << 6, w);
             if (nin == 0) e0 = e;
             else if (nin == 1) e1 = e;
             else bin[lane * xcap + nin - 2] = e;
@@ -1949,15 +1969,28 @@ __device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t*
       }
       drain_vmem();
     }
-    if (lane == 0) {
-      int len = 0;
-      int x = __hip_atomic_load(pre + A.rank[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      while (x >= 0 && A.rank_id[x] != 0) {
-        path[len++] = x;
-        x = __hip_atomic_load(pre + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The path back from the sink, in descending ranks (a choice always has a lower rank): the
+    // wave loads the choices and node ids of the 64-rank block holding the current rank at once
+    // and follows the path inside it across the lanes.
+    int len = 0;
+    int x = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(pre + A.rank[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    while (x >= 0 && x < n && len < n) {  // bounded even if a choice were not of lower rank
+      const int b = x & ~63;
+      const bool in = b + lane < n;
+      const int pv = in ? __hip_atomic_load(pre + b + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
+      const int id = in ? A.rank_id[b + lane] : 0;
+      while (x >= b && x < b + 64 && len < n) {
+        if (__builtin_amdgcn_readlane(id, x - b) == 0) {  // the source
+          x = -1;
+          break;
+        }
+        if (lane == 0) path[len] = x;
+        len++;
+        x = __builtin_amdgcn_readlane(pv, x - b);
       }
-      S.cons_len = len;
     }
+    if (lane == 0) S.cons_len = len;
   }
   __syncthreads();
   const int len = S.cons_len;
