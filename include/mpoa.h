@@ -160,6 +160,16 @@ int mpoa_batch_set_predictions(mpoa_batch* b, const float* pred, int32_t stride)
 enum { MPOA_MODE_BATCHED = 0, MPOA_MODE_PLAIN = 1 };
 int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
 
+/* Test hook (no counterpart in minipoa): runs every read of the batch through the fallback
+ * forms the library otherwise takes only for unusual graphs, so that tests can hold each of them
+ * to the same output. Bits: MPOA_FORM_SORT_HBM, the sort walk over a CSR in HBM with atomic
+ * degree counts (graphs too big for LDS); MPOA_FORM_SORT_LANE, the one-lane sort (a ready stack
+ * past 512 nodes); MPOA_FORM_BT_LANE, the one-lane backtrack (a row with more than 63
+ * predecessors); MPOA_FORM_ADD_LANE, the one-lane add_path (a repeated edge). 0 (the default)
+ * restores the normal choice. Results do not change; only speed does. */
+enum { MPOA_FORM_SORT_HBM = 1, MPOA_FORM_SORT_LANE = 2, MPOA_FORM_BT_LANE = 4, MPOA_FORM_ADD_LANE = 8 };
+int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms);
+
 /* Graph replay of the batched pass (default on): the concurrent persistent launches, each with
  * its work-queue reset, are captured once into a hipGraph and replayed by every later
  * mpoa_batch_execute whose launch arguments are unchanged; off, they are enqueued one by one.

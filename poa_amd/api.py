@@ -18,6 +18,7 @@ import numpy as np
 from ._lib import PRODUCT_SO
 
 MPOA_OK = 0
+FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE = 1, 2, 4, 8
 ERRORS = {
     -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
     -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW",
@@ -27,7 +28,7 @@ EXPORTS = [
     "mpoa_create", "mpoa_destroy", "mpoa_strerror", "mpoa_align_reads", "mpoa_batch_prepare",
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
-    "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph",
+    "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph", "mpoa_batch_set_forms",
     "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
@@ -118,6 +119,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_set_traces.argtypes = [VP, I32]
         L.mpoa_batch_set_mode.argtypes = [VP, I32]
         L.mpoa_batch_set_graph.argtypes = [VP, I32]
+        L.mpoa_batch_set_forms.argtypes = [VP, I32]
         L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
@@ -255,6 +257,12 @@ class Batch:
         """Replay the batched pass as a captured hipGraph (default) or enqueue its launches
         one by one."""
         _check(lib().mpoa_batch_set_graph(self.h, int(bool(enable))))
+
+    def set_forms(self, forms: int) -> None:
+        """Test hook: run every read through the fallback forms given by FORM_* bits (the sort
+        over a CSR in HBM, the one-lane sort, backtrack or add_path); 0 restores the normal
+        choice. Results do not change."""
+        _check(lib().mpoa_batch_set_forms(self.h, int(forms)))
 
     def set_predictions(self, pred: np.ndarray) -> None:
         """Per-window (cost, max call cells, max band width, final nodes) predictions that

@@ -211,6 +211,7 @@ struct mpoa_batch {
   std::vector<mpoa_launch_stat> lstats;  // launches of the last execute
   DevBuf<int64_t> d_gfa;                  // one-window runs: offsets of the GFA path lines
   int mode = MPOA_MODE_BATCHED;
+  int forms = 0;  // MPOA_FORM_* test hook
   // Learned predictions per window (mpoa_batch_set_predictions): DP cells of the costliest
   // call, widest band row and final node count. Empty when none were given.
   std::vector<float> pred_cells, pred_width, pred_nodes;
@@ -313,6 +314,7 @@ int upload_order(Bin& bin, hipStream_t stream) {
 // The persistent kernel's arguments for a provisioned bin.
 KernelArgs args_of(const mpoa_batch* b, const Bin& bin) {
   KernelArgs a{};
+  a.forms = b->forms;
   a.scores = b->sc;
   a.caps = bin.caps;
   a.read_seq = b->d_seq.p;
@@ -629,6 +631,7 @@ int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vecto
                      : b->bin_of[w] == 3 ? b->wide
                                          : b->small;
     KernelArgs a{};
+    a.forms = b->forms;
     a.scores = b->sc;
     a.caps = bin.caps;
     a.read_seq = b->d_seq.p;
@@ -705,6 +708,7 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
         (rc = g_dp.alloc(3 * 64)) != MPOA_OK || (rc = d_cons.alloc(3 * (size_t)gc.nodes)) != MPOA_OK)
       return rc;
     KernelArgs ga{};
+    ga.forms = b->forms;
     ga.scores = b->sc;
     ga.caps = gc;
     ga.read_seq = b->d_seq.p;
@@ -1245,6 +1249,13 @@ int mpoa_batch_set_graph(mpoa_batch* b, int32_t enable) {
 int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode) {
   if (!b || (mode != MPOA_MODE_BATCHED && mode != MPOA_MODE_PLAIN)) return MPOA_E_ARG;
   b->mode = mode;
+  return MPOA_OK;
+}
+
+int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms) {
+  constexpr int all = MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE | MPOA_FORM_BT_LANE | MPOA_FORM_ADD_LANE;
+  if (!b || (forms & ~all) != 0) return MPOA_E_ARG;
+  b->forms = forms;
   return MPOA_OK;
 }
 

@@ -80,6 +80,7 @@ struct KernelArgs {
   int64_t* gfa_lines;       // optional (one window): offset of each read's GFA path line
   CallStat* call_stats;     // optional: per DP call, window w's calls from call_off[w]
   const int64_t* call_off;  // n_windows + 1 offsets (reads - 1 per window)
+  int forms;                // MPOA_FORM_* bits: fallback forms taken for every read (0 in use)
 };
 
 // -S sub-window of the current read: the DP of read characters [q0, q0 + qlen) against the

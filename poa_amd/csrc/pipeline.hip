@@ -643,10 +643,12 @@ __device__ __noinline__ void tree_depth_blocked(int n, const T* par, T* dep, int
 
 template <int NT>
 __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps& c, bool with_len, uint8_t* lds,
-                         int lds_bytes) {
+                         int lds_bytes, int forms) {
   const int64_t tk = now();
-  if (!kahn_lds<NT>(A, S, lds, lds_bytes) && !kahn_mid<NT>(A, S, c, lds, lds_bytes) &&
-      !kahn_hbm<NT>(A, S, lds, lds_bytes)) {
+  const bool lds_forms = (forms & (MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE)) == 0;
+  if ((forms & MPOA_FORM_SORT_LANE) ||
+      (!(lds_forms && (kahn_lds<NT>(A, S, lds, lds_bytes) || kahn_mid<NT>(A, S, c, lds, lds_bytes))) &&
+       !kahn_hbm<NT>(A, S, lds, lds_bytes))) {
     if (threadIdx.x == 0) kahn_global(A, S);
     __syncthreads();
   }
@@ -1760,7 +1762,7 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
   // global store there would make the walk's next loads wait for it as well.
   int* log = m <= 2 * RING * WR ? reinterpret_cast<int*>(&R) : A.res_from;  // H, D first
   if (__builtin_amdgcn_readfirstlane(tid >> 6) == 0) {  // wave 0 (a wave-uniform branch)
-    const int rc = backtrack_wave(A, S, sc, log);
+    const int rc = (a.forms & MPOA_FORM_BT_LANE) ? 1 : backtrack_wave(A, S, sc, log);
     if (tid == 0) {
       // A row with more than 63 predecessors: the one-lane backtrack, which writes the
       // res_t entries itself.
@@ -2021,7 +2023,7 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
   }
   {
     const int64_t ta = now();
-    if (!add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
+    if ((a.forms & MPOA_FORM_ADD_LANE) || !add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
       if (tid == 0) add_path(A, S, a.caps, keep_path, r, m, path0);
     }
     __syncthreads();
@@ -2034,7 +2036,7 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
     // records; the consensus output does not read it
     if (tid == 0 && a.scores.result != 0) S.msa_cols = topsort1(A, S);
   } else {
-    topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes);
+    topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes, a.forms);
     if (seeding) heaviest_ranked<NT>(A, S, lds, lds_bytes);
   }
   if (tid == 0) S.ticks[PH_SORT] += now() - tb;

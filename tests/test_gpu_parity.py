@@ -146,3 +146,34 @@ def test_held_text_follow_up_calls(ctx):
     assert rc == 0 and text == want_a and scores == [c.score for c in calls_a]
     rc, need, text, _ = call(ra, need_a)  # delivered: the next call runs again
     assert rc == 0 and text == want_a
+
+
+@pytest.mark.parametrize("forms", [api.FORM_SORT_HBM, api.FORM_SORT_LANE, api.FORM_BT_LANE,
+                                   api.FORM_ADD_LANE, 15])
+def test_fallback_forms_match_oracle(ctx, forms):
+    """Every read through the forms taken otherwise only for unusual graphs: the sort over a CSR
+    in HBM with atomic degrees, the one-lane sort, backtrack and add_path (graph.cpp:59-123,
+    align.cpp:1572-1736, graph.cpp:170-235)."""
+    b = synth.config3(seed=25, n_windows=24)
+    for result in (0, 1):
+        p = api.Params(result=result)
+        _say("fallback forms", forms, "result", result)
+        bat = api.Batch(ctx, p, b.win_off, b.seq, b.seq_off)
+        bat.set_forms(forms)
+        bat.execute()
+        got = bat.results().texts
+        bat.set_forms(0)
+        bat.execute()
+        again = bat.results().texts
+        bat.free()
+        want = oracle_lib.batch(p, b.win_off, b.seq, b.seq_off)
+        assert got == want
+        assert again == want
+
+
+def test_fallback_forms_reject_unknown_bits(ctx):
+    b = synth.config3(seed=26, n_windows=2)
+    bat = api.Batch(ctx, api.Params(), b.win_off, b.seq, b.seq_off)
+    with pytest.raises(api.MpoaError):
+        bat.set_forms(16)
+    bat.free()
