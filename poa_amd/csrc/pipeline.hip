@@ -1606,12 +1606,13 @@ __device__ __forceinline__ void adaptive_update(const Arena& A, WinShared& S, in
 // LDS ring shape per workgroup size: RING rows of WR columns (int32 H and D).
 template <int NT>
 struct RingShape {
-  // 64: six rows of 344 columns, 20 KB in all, so that eight workgroups share a CU (the
-  // seventh and eighth most recent rows serve ~4% of the predecessor reads; concurrency
-  // matters more). 128/256: the larger batched tiles, sized so that the sort of a graph of up
-  // to ~5,400 / ~8,000 nodes runs in LDS while leaving room for small-tile workgroups on the
-  // same CU (39 / 63 KB). 512: one window alone (minipoa's own run), wide bands.
-  static constexpr int RING = NT == 64 ? 6 : RING_ROWS;
+  // 64: four rows of 344 columns, 14.3 KB in all, so that eleven workgroups share a CU (the
+  // DP is latency-bound: more windows in flight beat the predecessor reads the fifth to eighth
+  // most recent rows would serve from LDS). 128/256: the larger batched tiles, sized so that
+  // the sort of a graph of up to ~5,400 / ~8,000 nodes runs in LDS while leaving room for
+  // small-tile workgroups on the same CU (39 / 63 KB). 512: one window alone (minipoa's own
+  // run), wide bands.
+  static constexpr int RING = NT == 64 ? 4 : RING_ROWS;
   static constexpr int WR = NT == 64 ? 344 : NT == 128 ? 512 : NT == 256 ? 896 : 2048;
   // longest query held in LDS (64: queries of up to 1,500 bases)
   static constexpr int QCAP = NT == 64 ? 1536 : NT <= 256 ? 4096 : 24576;
