@@ -165,10 +165,17 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
  * to the same output. Bits: MPOA_FORM_SORT_HBM, the sort walk over a CSR in HBM with atomic
  * degree counts (graphs too big for LDS); MPOA_FORM_SORT_LANE, the one-lane sort (a ready stack
  * past 512 nodes); MPOA_FORM_BT_LANE, the one-lane backtrack (a row with more than 63
- * predecessors); MPOA_FORM_ADD_LANE, the one-lane add_path (a repeated edge). 0 (the default)
- * restores the normal choice. Results do not change; only speed does. */
-enum { MPOA_FORM_SORT_HBM = 1, MPOA_FORM_SORT_LANE = 2, MPOA_FORM_BT_LANE = 4, MPOA_FORM_ADD_LANE = 8 };
+ * predecessors); MPOA_FORM_ADD_LANE, the one-lane add_path (a repeated edge);
+ * MPOA_FORM_CONS_ROUNDS, the -S consensus settling every block in rounds (a rank with more than
+ * eight predecessors in its block). 0 (the default) restores the normal choice. Results do not
+ * change; only speed does. mpoa_set_forms sets the forms of the batches the context creates
+ * afterwards, mpoa_align_reads' included. */
+enum {
+  MPOA_FORM_SORT_HBM = 1, MPOA_FORM_SORT_LANE = 2, MPOA_FORM_BT_LANE = 4, MPOA_FORM_ADD_LANE = 8,
+  MPOA_FORM_CONS_ROUNDS = 16
+};
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms);
+int mpoa_set_forms(mpoa_ctx* ctx, int32_t forms);
 
 /* Graph replay of the batched pass (default on): the concurrent persistent launches, each with
  * its work-queue reset, are captured once into a hipGraph and replayed by every later

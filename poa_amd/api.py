@@ -18,7 +18,7 @@ import numpy as np
 from ._lib import PRODUCT_SO
 
 MPOA_OK = 0
-FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE = 1, 2, 4, 8
+FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE, FORM_CONS_ROUNDS = 1, 2, 4, 8, 16
 ERRORS = {
     -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
     -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW",
@@ -29,6 +29,7 @@ EXPORTS = [
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
     "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph", "mpoa_batch_set_forms",
+    "mpoa_set_forms",
     "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
@@ -120,6 +121,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_set_mode.argtypes = [VP, I32]
         L.mpoa_batch_set_graph.argtypes = [VP, I32]
         L.mpoa_batch_set_forms.argtypes = [VP, I32]
+        L.mpoa_set_forms.argtypes = [VP, I32]
         L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
@@ -164,6 +166,11 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_forms(self, forms: int) -> None:
+        """Test hook: the FORM_* fallback forms of every batch created afterwards, including
+        the one behind align_reads; 0 restores the normal choice. Results do not change."""
+        _check(lib().mpoa_set_forms(self.h, int(forms)))
 
     def align_reads(self, params: Params, seq: np.ndarray, off: np.ndarray,
                     names: list[str] | None = None, comments: list[str] | None = None,

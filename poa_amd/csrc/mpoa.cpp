@@ -42,6 +42,7 @@ struct mpoa_ctx {
   // with the same arguments
   std::string held_key, held_text;
   std::vector<mpoa_call_stat> held_calls;
+  int forms = 0;  // MPOA_FORM_* test hook: the forms of batches created afterwards
 };
 
 namespace {
@@ -978,6 +979,7 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
   HIPCHK(hipSetDevice(ctx->device));
   mpoa_batch* b = new mpoa_batch();
   b->ctx = ctx;
+  b->forms = ctx->forms;
   b->params = *params;
   b->sc = scores_of(*params);
   b->n_windows = n_windows;
@@ -1266,10 +1268,21 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode) {
   return MPOA_OK;
 }
 
+constexpr int kAllForms = MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE | MPOA_FORM_BT_LANE |
+                          MPOA_FORM_ADD_LANE | MPOA_FORM_CONS_ROUNDS;
+
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms) {
-  constexpr int all = MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE | MPOA_FORM_BT_LANE | MPOA_FORM_ADD_LANE;
-  if (!b || (forms & ~all) != 0) return MPOA_E_ARG;
+  if (!b || (forms & ~kAllForms) != 0) return MPOA_E_ARG;
   b->forms = forms;
+  return MPOA_OK;
+}
+
+int mpoa_set_forms(mpoa_ctx* ctx, int32_t forms) {
+  if (!ctx || (forms & ~kAllForms) != 0) return MPOA_E_ARG;
+  ctx->forms = forms;
+  ctx->held_key.clear();
+  ctx->held_text.clear();
+  ctx->held_calls.clear();
   return MPOA_OK;
 }
 

@@ -1834,7 +1834,8 @@ __device__ __forceinline__ void trivial_res(Arena& A, int m) {
 constexpr int HR_IN = 8;
 
 template <int NT>
-__device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
+__device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t* lds, int lds_bytes,
+                                                int forms) {
   const int n = S.n_rows;
   int* score = A.t2;
   int* pre = A.t1;
@@ -1895,6 +1896,7 @@ __device__ __forceinline__ void heaviest_ranked(Arena& A, WinShared& S, uint8_t*
           }
         }
       }
+      if (forms & MPOA_FORM_CONS_ROUNDS) reread = true;
       int sc = 0, pr = -1;
       if (!done && nin == 0) {
         sc = eps + ew;
@@ -2038,7 +2040,7 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
     if (tid == 0 && a.scores.result != 0) S.msa_cols = topsort1(A, S);
   } else {
     topsort0<NT>(A, S, a.caps, with_len, lds, lds_bytes, a.forms);
-    if (seeding) heaviest_ranked<NT>(A, S, lds, lds_bytes);
+    if (seeding) heaviest_ranked<NT>(A, S, lds, lds_bytes, a.forms);
   }
   if (tid == 0) S.ticks[PH_SORT] += now() - tb;
   __syncthreads();

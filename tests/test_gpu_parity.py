@@ -175,5 +175,20 @@ def test_fallback_forms_reject_unknown_bits(ctx):
     b = synth.config3(seed=26, n_windows=2)
     bat = api.Batch(ctx, api.Params(), b.win_off, b.seq, b.seq_off)
     with pytest.raises(api.MpoaError):
-        bat.set_forms(16)
+        bat.set_forms(32)
     bat.free()
+
+
+@pytest.mark.parametrize("forms", [api.FORM_CONS_ROUNDS, 31])
+def test_seeded_fallback_forms_match_oracle(ctx, forms):
+    """-S windows with the fallback forms forced through the context: the consensus settled in
+    rounds (graph.cpp:294-350), and with 31 every other fallback form as well."""
+    ctx.set_forms(forms)
+    try:
+        reads = synth.window(33, 1500, 10, 0.04, 0.03, 0.03, 0.7)
+        _say("seeded fallback forms", forms)
+        _check_window(ctx, reads, seeding=1)
+        reads = synth.window(34, 4000, 5, 0.024, 0.048, 0.048, 0.7)
+        _check_window(ctx, reads, seeding=1, k=15, mm_w=10, result=1)
+    finally:
+        ctx.set_forms(0)
