@@ -776,55 +776,6 @@ __device__ __forceinline__ int topsort1(Arena& A, WinShared& S) {
   return r;
 }
 
-// State of the heaviest-bundle walk of the consensus output, set by all threads:
-// in-degrees in t0, scores in t2, predecessors in pre. The caller synchronises before the walk.
-template <int NT>
-__device__ __forceinline__ void heaviest_init(Arena& A, const WinShared& S, int* pre) {
-  for (int v = threadIdx.x; v < S.n_nodes; v += NT) {
-    A.t0[v] = A.in_cnt[v];
-    A.t2[v] = 0;
-    pre[v] = 0;
-  }
-}
-
-// graph::build_consensus (src/graph.cpp:294-350): FIFO BFS from the source; each node keeps
-// the predecessor with the largest in-weight, ties to the larger accumulated score. Writes the
-// consensus bases to out (reversed trace) and returns its length, or -1 if over cap.
-__device__ __forceinline__ int consensus(Arena& A, WinShared& S, uint8_t* out, int cap) {
-  int* deg = A.t0;  // in-degrees, scores and predecessors set by heaviest_init
-  int* queue = A.t1;
-  int* score = A.t2;
-  int* pre = A.hlen;
-  int qh = 0, qt = 0;
-  queue[qt++] = 0;
-  while (qh < qt) {
-    int u = queue[qh++];
-    if (u != 0) {
-      int wmax = -1, max_pre = -1;
-      for (int e = A.in_head[u]; e != -1; e = A.ie_next[e]) {
-        int v = A.ie_src[e], w = A.ie_w[e];
-        if (w > wmax || (w == wmax && score[v] > score[max_pre])) {
-          wmax = w;
-          max_pre = v;
-        }
-      }
-      pre[u] = max_pre;
-      score[u] = score[max_pre] + wmax;
-    }
-    if (u == 1) break;
-    for (int e = A.out_head[u]; e != -1; e = A.oe_next[e]) {
-      int v = A.oe_dst[e];
-      if (--deg[v] == 0) queue[qt++] = v;
-    }
-  }
-  int len = 0;
-  for (int id = pre[1]; id != 0; id = pre[id]) len++;
-  if (len > cap) return -1;
-  int at = len;
-  for (int id = pre[1]; id != 0; id = pre[id]) out[--at] = char_of(A.base[id]);
-  return len;
-}
-
 // ---------------------------------------------------------------------------------------
 // Banded DP (src/align.cpp:1357-1514).
 // ---------------------------------------------------------------------------------------
@@ -2124,18 +2075,24 @@ __device__ int64_t gfa_text(const KernelArgs& a, Arena& A, WinShared& S, const W
 
 // Output text of a finished window and its result record.
 template <int NT>
-__device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int w) {
+__device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, WinShared& S, int w,
+                                               uint8_t* lds, int lds_bytes) {
   const WinDesc& wd = a.win[w];
   const int tid = threadIdx.x;
   const int64_t t_out = now();
   if (S.status == MPOA_OK) {
     if (a.scores.result == 0) {
-      heaviest_init<NT>(A, S, A.hlen);
-      __syncthreads();
-      if (tid == 0) {
-        const int len = consensus(A, S, a.out + wd.out0, wd.out_cap);
-        if (len < 0) S.status = MPOA_E_OVERFLOW;
-        else S.cons_len = len;
+      // graph::output_consensus (graph.cpp:294-357): the last read left the graph unsorted, so
+      // the wave sort gives the rank order and predecessor CSR the blocked heaviest-bundle walk
+      // reads (any topological order gives minipoa's FIFO result), then the bases are written.
+      topsort0<NT>(A, S, a.caps, false, lds, lds_bytes, a.forms);
+      heaviest_ranked<NT>(A, S, lds, lds_bytes, a.forms);
+      const int len = S.cons_len;
+      if (len > wd.out_cap) {
+        if (tid == 0) S.status = MPOA_E_OVERFLOW;
+      } else {
+        uint8_t* o = a.out + wd.out0;
+        for (int at = tid; at < len; at += NT) o[at] = char_of(A.cons_ch[at]);
       }
       __syncthreads();
     } else if (a.scores.result == 1) {
@@ -2256,7 +2213,7 @@ __device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinSha
     __syncthreads();
     if (S.status != MPOA_OK) break;
   }
-  window_outputs<NT>(a, A, S, w);
+  window_outputs<NT>(a, A, S, w, reinterpret_cast<uint8_t*>(&R), (int)sizeof(R));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -2414,7 +2371,8 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
     }
     __syncthreads();
   }
-  if (r + 1 == s.n_reads || S.status != MPOA_OK) window_outputs<NT>(a, A, S, 0);
+  if (r + 1 == s.n_reads || S.status != MPOA_OK)
+    window_outputs<NT>(a, A, S, 0, reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
   if (tid == 0) state_store(S, *s.state);
 }
 
