@@ -483,7 +483,11 @@ __device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off
             vo0 = e.y;
             vo1 = e.z;
           } else {
+            // the target's edge range first: read before the degree store in program order,
+            // its loads go out with the degree's instead of after the store
             v = odst[o0 + k];
+            vo0 = ooff[v];
+            vo1 = ooff[v + 1];
           }
           int d;
           if constexpr (HBM) {
@@ -493,10 +497,6 @@ __device__ __forceinline__ bool kahn_walk(Arena& A, WinShared& S, Off* ooff, Off
             deg[v] = (Deg)d;
           }
           z = d == 0;
-          if constexpr (!REC) {
-            vo0 = ooff[v];
-            vo1 = ooff[v + 1];
-          }
         }
         const uint64_t mk = __ballot(z);
         if (sp + __popcll(mk) > KAHN_STK) {
