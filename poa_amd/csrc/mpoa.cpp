@@ -26,8 +26,6 @@ using mpoa::SeedArgs;
 using mpoa::SegDesc;
 using mpoa::SegResult;
 using mpoa::WinState;
-using mpoa::slot_bytes;
-using mpoa::slot_ints;
 namespace seed = mpoa::seed;
 
 struct mpoa_ctx {
@@ -260,8 +258,7 @@ SlotCaps make_caps(const std::vector<Estimate>& est, const std::vector<WinInfo>&
   c.nodes = (c.nodes + 63) / 64 * 64;
   c.edges = (c.edges + 63) / 64 * 64;
   c.dp = (c.dp + 63) / 64 * 64;
-  c.ints_per_slot = (mpoa::slot_ints(c.nodes, c.edges, c.qmax, c.path) + 63) / 64 * 64;
-  c.bytes_per_slot = (mpoa::slot_bytes(c.nodes, c.edges, c.qmax) + 255) / 256 * 256;
+  mpoa::finish_caps(c);
   return c;
 }
 
@@ -716,8 +713,7 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
     gc.qmax = std::max(1, wi.qmax);
     gc.path = keep_path ? (int)wi.qtot : 0;
     gc.dp = 64;
-    gc.ints_per_slot = (slot_ints(gc.nodes, gc.edges, gc.qmax, gc.path) + 63) / 64 * 64;
-    gc.bytes_per_slot = (slot_bytes(gc.nodes, gc.edges, gc.qmax) + 255) / 256 * 256;
+    mpoa::finish_caps(gc);
     if ((rc = g_int.alloc((size_t)gc.ints_per_slot)) != MPOA_OK ||
         (rc = g_byte.alloc((size_t)gc.bytes_per_slot)) != MPOA_OK ||
         (rc = g_dp.alloc(3 * 64)) != MPOA_OK || (rc = d_cons.alloc(3 * (size_t)gc.nodes)) != MPOA_OK)
@@ -836,10 +832,7 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
           sc_caps.edges = 64;
           sc_caps.path = 0;
           sc_caps.dp = std::max<int64_t>(dp_need, seg_dp);
-          sc_caps.ints_per_slot =
-              (slot_ints(sc_caps.nodes, sc_caps.edges, sc_caps.qmax, 0) + 63) / 64 * 64;
-          sc_caps.bytes_per_slot = (slot_bytes(sc_caps.nodes, sc_caps.edges, sc_caps.qmax) + 255) /
-                                   256 * 256;
+          mpoa::finish_caps(sc_caps);
           const size_t slots = std::max<size_t>((size_t)nseg, seg_slots);
           HIPCHK(hipStreamSynchronize(ctx->stream));
           if ((rc = s_int.alloc(slots * sc_caps.ints_per_slot)) != MPOA_OK ||

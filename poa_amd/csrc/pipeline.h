@@ -28,12 +28,88 @@ struct SlotCaps {
   int64_t bytes_per_slot;
 };
 
-// Int32 and byte words a slot's arena takes; must match carve() in pipeline.hip.
+// Int32 and byte words a slot's arena takes: carve() below stays within them (checked on the
+// host for many capacities by tests/carve_check.cpp).
 inline __host__ __device__ int64_t slot_ints(int N, int E, int qmax, int path) {
   return 46LL * N + 8LL * E + 2LL * qmax + 8 + path;
 }
 inline __host__ __device__ int64_t slot_bytes(int N, int E, int qmax) {
   return 2LL * N + E + 3LL * qmax + 128;
+}
+
+// Slot sizes of caps whose nodes, edges, qmax, path and dp are set: whole 256-byte blocks.
+inline void finish_caps(SlotCaps& c) {
+  c.ints_per_slot = (slot_ints(c.nodes, c.edges, c.qmax, c.path) + 63) / 64 * 64;
+  c.bytes_per_slot = (slot_bytes(c.nodes, c.edges, c.qmax) + 255) / 256 * 256;
+}
+
+constexpr int NB = 5;         // alphabet incl. N (minipoa's m = 5, src/handle_input.cpp:51)
+constexpr int QPAD = 32;      // query bytes past qmax: the N padding to col_size plus 2*Q
+constexpr int RING_ROWS = 8;  // rows kept in the LDS ring (tiles above 64 threads)
+
+// Per-slot arena: carved from the int pool / byte pool / DP pool at fixed offsets. Every
+// device function that takes the arena is force-inlined: one left as a call needs the arena's
+// address, which moves the whole structure to scratch memory and turns each of its pointers
+// into a scratch load inside the DP, sort and backtrack loops.
+struct Arena {
+  uint8_t* base;
+  int *par, *aligned, *ind, *rank, *in_head, *in_tail, *out_head, *out_tail, *in_cnt, *last_read;
+  int* out_cnt;
+  int *ie_src, *ie_w, *ie_next, *oe_dst, *oe_w, *oe_next;
+  int *rank_id, *t0, *t1, *t2, *hlen, *tlen;
+  int *pred_off, *pred_rank, *pred_w, *Bs, *Be, *row_off;
+  uint8_t* pred_base;
+  int* rowrec;  // RowRec per rank (12 ints)
+  int *Mp, *Sl, *Pl, *Pr, *Ol, *Or, *Ow;
+  int *res_from, *res_aligned;
+  int *cons_id, *cons_rank;  // consensus path for -S seeding
+  uint8_t *res_base, *qseq, *qsh;
+  uint8_t* cons_ch;
+  int* path;
+  int *H, *D, *I;
+  int* int_end;       // one past the last int word taken
+  uint8_t* byte_end;  // one past the last byte taken
+};
+
+// Slot `slot` of the three pools. Pointer arithmetic only (no access), so the host can check
+// the layout against slot_ints / slot_bytes. The int arrays start with the row records (int4
+// accesses: ints_per_slot is a multiple of 64); qseq and qsh are 16-byte aligned.
+__host__ __device__ inline Arena carve_pools(const SlotCaps& c, int* int_pool, uint8_t* byte_pool,
+                                             int* dp_pool, int slot) {
+  Arena A;
+  int* p = int_pool + (int64_t)slot * c.ints_per_slot;
+  auto take = [&](int64_t n) { int* r = p; p += n; return r; };
+  const int N = c.nodes, E = c.edges;
+  A.rowrec = take(12LL * N);
+  A.par = take(N); A.aligned = take((int64_t)NB * N); A.ind = take(N); A.rank = take(N);
+  A.in_head = take(N); A.in_tail = take(N); A.out_head = take(N); A.out_tail = take(N);
+  A.in_cnt = take(N); A.last_read = take(N); A.out_cnt = take(N);
+  A.ie_src = take(E); A.ie_w = take(E); A.ie_next = take(E);
+  A.oe_dst = take(E); A.oe_w = take(E); A.oe_next = take(E);
+  A.rank_id = take(N); A.t0 = take(N); A.t1 = take(N); A.t2 = take(N);
+  A.hlen = take(N); A.tlen = take(N);
+  A.pred_off = take(N + 1); A.pred_rank = take(E); A.pred_w = take(E);
+  A.Bs = take(N); A.Be = take(N); A.row_off = take(N + 1);
+  A.Mp = take(N); A.Sl = take(N); A.Pl = take(N); A.Pr = take(N);
+  A.Ol = take(N); A.Or = take(N); A.Ow = take(N);
+  A.res_from = take(c.qmax + 2); A.res_aligned = take(c.qmax + 2);
+  A.path = take(c.path);
+  A.cons_id = take(N); A.cons_rank = take(N);
+  A.int_end = p;
+  uint8_t* b = byte_pool + (int64_t)slot * c.bytes_per_slot;
+  A.base = b; b += N;
+  A.cons_ch = b; b += N;
+  A.pred_base = b; b += E;
+  A.res_base = b; b += c.qmax + 2;
+  auto align16 = [](uint8_t* x) {
+    return reinterpret_cast<uint8_t*>((reinterpret_cast<uintptr_t>(x) + 15) & ~uintptr_t(15));
+  };
+  A.qseq = align16(b); b = A.qseq + c.qmax + QPAD;
+  A.qsh = align16(b); b = A.qsh + c.qmax + QPAD;
+  A.byte_end = b;
+  int* dp = dp_pool + (int64_t)slot * 3 * c.dp;
+  A.H = dp; A.D = dp + c.dp; A.I = dp + 2 * c.dp;
+  return A;
 }
 
 struct WinDesc {
@@ -116,6 +192,10 @@ struct SeedArgs {
   int* cons_out;           // after a commit: (node id, rank, base) per consensus position
   int cons_cap;
 };
+
+__host__ __device__ inline Arena carve(const KernelArgs& a, int slot) {
+  return carve_pools(a.caps, a.int_pool, a.byte_pool, a.dp_pool, slot);
+}
 
 }  // namespace mpoa
 

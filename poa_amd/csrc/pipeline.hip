@@ -42,67 +42,6 @@ __device__ __forceinline__ uint8_t char_of(int code) {
   return (uint8_t)(code == 0 ? 'A' : code == 1 ? 'C' : code == 2 ? 'G' : code == 3 ? 'T' : 'N');
 }
 
-// Per-slot arena: carved from the int pool / byte pool / DP pool at fixed offsets. Every
-// device function that takes the arena is force-inlined: one left as a call needs the arena's
-// address, which moves the whole structure to scratch memory and turns each of its pointers
-// into a scratch load inside the DP, sort and backtrack loops.
-constexpr int RING_ROWS = 8;  // rows kept in the LDS ring (tiles above 64 threads)
-
-struct Arena {
-  uint8_t* base;
-  int *par, *aligned, *ind, *rank, *in_head, *in_tail, *out_head, *out_tail, *in_cnt, *last_read;
-  int* out_cnt;
-  int *ie_src, *ie_w, *ie_next, *oe_dst, *oe_w, *oe_next;
-  int *rank_id, *t0, *t1, *t2, *hlen, *tlen;
-  int *pred_off, *pred_rank, *pred_w, *Bs, *Be, *row_off;
-  uint8_t* pred_base;
-  int* rowrec;  // RowRec per rank (12 ints)
-  int *Mp, *Sl, *Pl, *Pr, *Ol, *Or, *Ow;
-  int *res_from, *res_aligned;
-  int *cons_id, *cons_rank;  // consensus path for -S seeding
-  uint8_t *res_base, *qseq, *qsh;
-  uint8_t* cons_ch;
-  int* path;
-  int *H, *D, *I;
-};
-
-__device__ Arena carve(const KernelArgs& a, int slot) {
-  const SlotCaps& c = a.caps;
-  Arena A;
-  int* p = a.int_pool + (int64_t)slot * c.ints_per_slot;
-  auto take = [&](int64_t n) { int* r = p; p += n; return r; };
-  const int N = c.nodes, E = c.edges;
-  A.rowrec = take(12LL * N);  // first: 16-byte aligned for the int4 accesses
-  A.par = take(N); A.aligned = take((int64_t)NB * N); A.ind = take(N); A.rank = take(N);
-  A.in_head = take(N); A.in_tail = take(N); A.out_head = take(N); A.out_tail = take(N);
-  A.in_cnt = take(N); A.last_read = take(N); A.out_cnt = take(N);
-  A.ie_src = take(E); A.ie_w = take(E); A.ie_next = take(E);
-  A.oe_dst = take(E); A.oe_w = take(E); A.oe_next = take(E);
-  A.rank_id = take(N); A.t0 = take(N); A.t1 = take(N); A.t2 = take(N);
-  A.hlen = take(N); A.tlen = take(N);
-  A.pred_off = take(N + 1); A.pred_rank = take(E); A.pred_w = take(E);
-  A.Bs = take(N); A.Be = take(N); A.row_off = take(N + 1);
-  A.Mp = take(N); A.Sl = take(N); A.Pl = take(N); A.Pr = take(N);
-  A.Ol = take(N); A.Or = take(N); A.Ow = take(N);
-  A.res_from = take(c.qmax + 2); A.res_aligned = take(c.qmax + 2);
-  A.path = take(c.path);
-  A.cons_id = take(N); A.cons_rank = take(N);
-  uint8_t* b = a.byte_pool + (int64_t)slot * c.bytes_per_slot;
-  A.base = b; b += N;
-  A.cons_ch = b; b += N;
-  A.pred_base = b; b += E;
-  A.res_base = b; b += c.qmax + 2;
-  auto align16 = [](uint8_t* x) {
-    return reinterpret_cast<uint8_t*>((reinterpret_cast<uintptr_t>(x) + 15) & ~uintptr_t(15));
-  };
-  A.qseq = align16(b); b = A.qseq + c.qmax + 32;
-  A.qsh = align16(b);
-  int* dp = a.dp_pool + (int64_t)slot * 3 * c.dp;
-  // The int words taken above never exceed slot_ints(N, E, qmax, path) (pipeline.h).
-  A.H = dp; A.D = dp + c.dp; A.I = dp + 2 * c.dp;
-  return A;
-}
-
 // Window-level scalar state shared by the workgroup.
 struct WinShared {
   int n_nodes, n_in_e, n_out_e;
@@ -226,17 +165,39 @@ __device__ __forceinline__ void add_path(Arena& A, WinShared& S, const SlotCaps&
   add_adj(A, S, c, keep_path, seq_id, anchor, 1, pos, path0);
 }
 
-// add_path over all lanes. A read's path visits each node at most once, so every in-list and
-// out-list touched by one read belongs to a distinct node and the edits are independent; new
-// nodes take ids in res order (a prefix sum), exactly as the serial walk assigns them. A read
-// whose res would create the same aligned node twice falls back to the serial walk. Returns
-// false when that fallback is needed (nothing has been modified then).
+// add_path over all lanes. When a read's path visits each node at most once (a run of the
+// same node, X -> X, is one visit: add_adj skips self-loops), every in-list and out-list
+// touched by one read belongs to a distinct node and the edits are independent; new nodes take
+// ids in res order (a prefix sum), exactly as the serial walk assigns them. A read whose res
+// reuses an existing node at two positions apart, or would create the same aligned node twice,
+// falls back to the serial walk. Returns false when that fallback is needed (nothing has been
+// modified then). stamp: N ints of scratch.
 template <int NT>
 __device__ __forceinline__ bool add_path_parallel(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path,
-                                  int seq_id, int m, int path0, int* cur) {
+                                  int seq_id, int m, int path0, int* cur, int* stamp) {
+  bool dup = false;
+  // 0. existing node of each res entry (its own id, or the aligned node it reuses), stamped
+  //    with the position of the run it opens; a second run of the same node finds a stamp
+  {
+    auto reused = [&](int r) {
+      const int f = A.res_from[r];
+      if (f >= 0) return f;
+      const int al = A.res_aligned[r];
+      return al >= 0 ? A.aligned[al * NB + A.res_base[r]] : -1;
+    };
+    for (int r = threadIdx.x; r < m; r += NT) {
+      const int v = reused(r);
+      if (v >= 0 && (r == 0 || reused(r - 1) != v)) stamp[v] = -1;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < m; r += NT) {
+      const int v = reused(r);
+      if (v >= 0 && (r == 0 || reused(r - 1) != v) && atomicExch(&stamp[v], r) != -1) dup = true;
+    }
+    if (__syncthreads_or(dup)) return false;
+  }
   // 1. claim empty aligned slots (key al*NB+b) for the earliest res entry: atomicMin of
   //    INT32_MIN + r (every claim is below -1, the empty mark; node ids are >= 0)
-  bool dup = false;
   for (int r = threadIdx.x; r < m; r += NT) {
     if (A.res_from[r] < 0 && A.res_aligned[r] >= 0) {
       const int key = A.res_aligned[r] * NB + A.res_base[r];
@@ -1977,7 +1938,7 @@ __device__ __forceinline__ void commit_read(const KernelArgs& a, Arena& A, WinSh
   }
   {
     const int64_t ta = now();
-    if ((a.forms & MPOA_FORM_ADD_LANE) || !add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2)) {
+    if ((a.forms & MPOA_FORM_ADD_LANE) || !add_path_parallel<NT>(A, S, a.caps, keep_path, r, m, path0, A.t2, A.t1)) {
       if (tid == 0) add_path(A, S, a.caps, keep_path, r, m, path0);
     }
     __syncthreads();

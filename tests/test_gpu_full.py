@@ -15,7 +15,8 @@ CASES = fu.load_cases()
 @pytest.mark.parametrize("case", CASES, ids=[c["case"] for c in CASES])
 def test_device_full_config(ctx, case):
     reads = fu.reads_of(case)
-    _, text = fu.fasta_of(reads)
+    sha, text = fu.fasta_of(reads)
+    assert sha == case["fasta_sha256"], "the generator's reads differ from the fixture's"
     seq, off, names, comments = gu.parse_fasta(text)
     print(f"full {case['case']} {' '.join(case['args'])}", flush=True)
     out, stats = ctx.align_reads(api.Params(**gu.parse_args(case["args"])), seq, off, names,
