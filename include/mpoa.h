@@ -39,8 +39,11 @@ enum {
   MPOA_E_EMPTY_WINDOW = -5, /* -S sub-window ends before it begins (minipoa: SIGSEGV) */
   MPOA_E_OOM = -6,          /* device allocation failed (minipoa: throw, mem_alloc_utils.cpp:27) */
   MPOA_E_DEVICE = -7,       /* HIP runtime error, or no GPU */
-  MPOA_E_OVERFLOW = -8      /* internal: a window outgrew its pre-sized arena; the library
-                               re-runs it with exact sizes and never returns this */
+  MPOA_E_OVERFLOW = -8,     /* a window outgrew its pre-sized arena: internal to the other
+                               entry points (they re-run it with exact sizes and never return
+                               this); reported by mpoa_align_reads_noalloc */
+  MPOA_E_SCRATCH = -9       /* mpoa_align_reads_noalloc: the window needs more than the fixed
+                               device scratch (MPOA_NOALLOC_SCRATCH_BYTES) */
 };
 
 typedef struct mpoa_ctx mpoa_ctx;
@@ -83,6 +86,32 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
                      const char* comment, const int64_t* comment_off, char* out,
                      int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
                      int32_t stats_cap, int32_t* n_stats);
+
+/* Allocation-free form of mpoa_align_reads. No counterpart in minipoa: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). One window, no seeding (params->seeding must be 0), consensus (-r 0) or
+ * row-column MSA (-r 1) output; the same kernels and results as mpoa_align_reads.
+ *
+ * Inputs and outputs are device buffers the caller owns: d_seq holds the reads' bases (ASCII),
+ * d_seq_off their n_reads + 1 offsets; seq_off is the same offsets in host memory (they size
+ * the scratch). d_out (out_cap bytes) receives the consensus bases (result 0) or the n_reads
+ * MSA rows back to back, each of the same length L (result 1): minipoa's text without its
+ * header lines. d_res (two int64 in device memory) receives the window's status (an MPOA_E_*
+ * code, 0 on success) and the output length in bytes.
+ *
+ * Every other byte the call uses (graph, DP matrices, queue and result records) lies in one
+ * fixed device array of MPOA_NOALLOC_SCRATCH_BYTES per device, held until the process ends;
+ * nothing is allocated, freed or copied by the host. A window whose capacity estimate does not
+ * fit the array is rejected with MPOA_E_SCRATCH before any launch; one that outgrows the
+ * estimate on the device reports MPOA_E_OVERFLOW in d_res (mpoa_align_reads runs it). Output
+ * longer than out_cap reports MPOA_E_OVERFLOW too. Asynchronous: the work is enqueued on
+ * `stream` (a hipStream_t; NULL: the context's stream) and d_res is valid once it completes.
+ * Every call on a device shares that device's array, which keeps the last call's contents: the
+ * kernels clear what must start empty, and calls that overlap on one device race. */
+#define MPOA_NOALLOC_SCRATCH_BYTES (256u << 20)
+int mpoa_align_reads_noalloc(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads,
+                             const char* d_seq, const int64_t* d_seq_off, const int64_t* seq_off,
+                             char* d_out, int64_t out_cap, int64_t* d_res, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Batched windows. minipoa has no batch interface: it aligns one read set per process

@@ -41,6 +41,7 @@ struct mpoa_ctx {
   std::string held_key, held_text;
   std::vector<mpoa_call_stat> held_calls;
   int forms = 0;  // MPOA_FORM_* test hook: the forms of batches created afterwards
+  uint8_t* scratch = nullptr;  // this device's allocation-free scratch array (device address)
 };
 
 namespace {
@@ -913,6 +914,8 @@ const char* mpoa_strerror(int rc) {
     case MPOA_E_OOM: return g_err.empty() ? "device out of memory" : g_err.c_str();
     case MPOA_E_DEVICE: return g_err.empty() ? "HIP device error" : g_err.c_str();
     case MPOA_E_OVERFLOW: return "arena overflow";
+    case MPOA_E_SCRATCH:
+      return g_err.empty() ? "window exceeds the allocation-free scratch" : g_err.c_str();
     default: return "unknown error";
   }
 }
@@ -937,6 +940,16 @@ int mpoa_create(int device, mpoa_ctx** out) {
     g_err = "device initialisation failed";
     return MPOA_E_DEVICE;
   }
+  void* scr = nullptr;
+  if (mpoa_scratch_address(device, &scr) != 0) {
+    (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream2);
+    (void)hipStreamDestroy(c->stream3);
+    delete c;
+    g_err = "device scratch lookup failed";
+    return MPOA_E_DEVICE;
+  }
+  c->scratch = static_cast<uint8_t*>(scr);
   c->n_cu = prop.multiProcessorCount;
   c->mem_total = prop.totalGlobalMem;
   int khz = 0;
@@ -1537,6 +1550,100 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   }
   mpoa_batch_free(b);
   return deliver(text, calls);
+}
+
+int mpoa_align_reads_noalloc(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads,
+                             const char* d_seq, const int64_t* d_seq_off, const int64_t* seq_off,
+                             char* d_out, int64_t out_cap, int64_t* d_res, void* stream) {
+  if (!ctx || !params || n_reads < 0 || !d_seq_off || !seq_off || !d_out || out_cap < 0 ||
+      !d_res || (n_reads > 0 && !d_seq))
+    return MPOA_E_ARG;
+  if (params->seeding) {
+    g_err = "seeding (-S) windows are not supported by the allocation-free form";
+    return MPOA_E_ARG;
+  }
+  if (params->result != 0 && params->result != 1) {
+    g_err = "the allocation-free form writes consensus (0) or MSA (1) output";
+    return MPOA_E_ARG;
+  }
+  if (!ctx->scratch) return MPOA_E_DEVICE;
+  HIPCHK(hipSetDevice(ctx->device));
+  const DpScores sc = scores_of(*params);
+  WinInfo wi{n_reads, 0, 0};
+  for (int r = 0; r < n_reads; r++) {
+    const int64_t l = seq_off[r + 1] - seq_off[r];
+    if (l < 0) return MPOA_E_ARG;
+    wi.qmax = (int)std::max<int64_t>(wi.qmax, l);
+    wi.qtot += l;
+  }
+  const bool keep_path = params->result != 0;
+  const bool full = params->ab_band != 0 && params->band_f > 0;
+  // Scratch layout: window records in the first 4 KB, then the int, byte and DP pools of one
+  // slot, each on a 256-byte boundary.
+  constexpr size_t kHead = 4096;
+  auto layout_bytes = [&](const SlotCaps& c) {
+    return kHead + (size_t)c.ints_per_slot * 4 + (size_t)c.bytes_per_slot + (size_t)c.dp * 12;
+  };
+  auto caps_of = [&](const Estimate& e) {
+    SlotCaps c{};
+    c.nodes = (std::max(e.nodes, 2) + 63) / 64 * 64;
+    c.edges = (std::max(e.edges, 4) + 63) / 64 * 64;
+    c.qmax = std::max(1, wi.qmax);
+    c.path = keep_path ? (int)wi.qtot : 0;
+    c.dp = (std::max<int64_t>(e.dp, 64) + 63) / 64 * 64;
+    mpoa::finish_caps(c);
+    return c;
+  };
+  // The read-length estimate, widened towards the worst case as far as the array allows.
+  const Estimate est = estimate(wi, sc, full), worst = worst_case(wi);
+  SlotCaps caps = caps_of(est);
+  if (layout_bytes(caps) > MPOA_NOALLOC_SCRATCH_BYTES) {
+    g_err = "window needs " + std::to_string(layout_bytes(caps)) +
+            " bytes of device scratch; the allocation-free form has " +
+            std::to_string((size_t)MPOA_NOALLOC_SCRATCH_BYTES);
+    return MPOA_E_SCRATCH;
+  }
+  for (double grow : {4.0, 2.0, 1.5}) {
+    Estimate e;
+    e.nodes = (int)std::min<double>(worst.nodes, grow * est.nodes);
+    e.edges = (int)std::min<double>(worst.edges, grow * est.edges);
+    e.dp = (int64_t)std::min<double>((double)worst.dp, grow * (double)est.dp);
+    const SlotCaps c = caps_of(e);
+    if (layout_bytes(c) <= MPOA_NOALLOC_SCRATCH_BYTES) {
+      caps = c;
+      break;
+    }
+  }
+  uint8_t* base = ctx->scratch;
+  KernelArgs a{};
+  a.forms = ctx->forms;
+  a.scores = sc;
+  a.caps = caps;
+  a.read_seq = reinterpret_cast<const uint8_t*>(d_seq);
+  a.read_off = d_seq_off;
+  a.win = reinterpret_cast<const WinDesc*>(base);
+  a.result = reinterpret_cast<WinResult*>(base + 256);
+  a.order = reinterpret_cast<const int*>(base + 2048);
+  a.n_jobs = 1;
+  a.queue = reinterpret_cast<int*>(base + 2304);
+  a.int_pool = reinterpret_cast<int*>(base + kHead);
+  a.byte_pool = base + kHead + (size_t)caps.ints_per_slot * 4;
+  a.dp_pool = reinterpret_cast<int*>(a.byte_pool + caps.bytes_per_slot);
+  a.out = reinterpret_cast<uint8_t*>(d_out);
+  static_assert(sizeof(WinDesc) <= 256 && sizeof(WinResult) <= 2048 - 256, "scratch head");
+  WinDesc wd{};
+  wd.read0 = 0;
+  wd.n_reads = n_reads;
+  wd.out0 = 0;
+  wd.out_cap = (int)std::min<int64_t>(out_cap, INT32_MAX / 2);
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  // one window alone: the tile mpoa_align_reads gives it
+  const int rc = mpoa_launch_noalloc(a, wd, 512, 2, d_res, st);
+  if (rc != 0) {
+    g_err = "kernel launch failed";
+    return MPOA_E_DEVICE;
+  }
+  return MPOA_OK;
 }
 
 }  // extern "C"

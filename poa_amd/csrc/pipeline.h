@@ -218,3 +218,12 @@ int mpoa_graph_add_hold(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
 // read (one workgroup). 0, -1 on a launch error, -2 for an unsupported tile.
 int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream);
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream);
+// Device address of this device's allocation-free scratch array (MPOA_NOALLOC_SCRATCH_BYTES),
+// looked up once per device with that device current. 0, or -1 when the lookup fails.
+int mpoa_scratch_address(int device, void** out);
+// The allocation-free window: descriptor and queue set-up, the persistent kernel with one slot,
+// and the copy of status and output length to d_res[0..1]; every pointer in `a` points into
+// the scratch array or the caller's buffers. 0, -1 on a launch error, -2 for an unsupported
+// tile.
+int mpoa_launch_noalloc(const mpoa::KernelArgs& a, const mpoa::WinDesc& wd, int nt, int cpl,
+                        int64_t* d_res, hipStream_t stream);

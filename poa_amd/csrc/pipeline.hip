@@ -17,13 +17,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "pipeline.h"
 
 namespace mpoa {
 
 constexpr int Q = 8;                      // band quantum (src/align.cpp:13,19)
 constexpr int NEG_INF = (int)0xc0c0c0c0;  // src/align.cpp:22
-constexpr int NB = 5;
 constexpr int CODE_N = 4;
 constexpr int M_OP = 1, D_OP = 2, I_OP = 4, ALL_OP = 7;
 constexpr int SCAN_ID = INT32_MIN;        // identity of the prefix max
@@ -2366,7 +2367,63 @@ __global__ void k_hold(int64_t ticks) {
   for (int i = 0; i < (1 << 16) && now() - t0 < ticks; i++) __builtin_amdgcn_s_sleep(64);
 }
 
+// ---------------------------------------------------------------------------------------
+// Allocation-free form (mpoa_align_reads_noalloc): one window whose work-queue counter,
+// processing order, descriptor, result record and slot arenas all live in `scratch`, a fixed
+// device array (each device holds its own copy until the process ends). Host code never uses
+// the array's name: it takes the address hipGetSymbolAddress returns for the current device
+// (mpoa_scratch_address) and passes pointers into it to the kernels below and to k_windows.
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) uint8_t scratch[MPOA_NOALLOC_SCRATCH_BYTES];
+
+// Writes the window descriptor and resets the processing order and the queue counter: kernel
+// arguments, not host copies, so that the whole sequence allocates nothing and may be captured.
+__global__ void k_noalloc_setup(WinDesc* wd, WinDesc v, int* order, int* queue) {
+  if (threadIdx.x == 0) {
+    *wd = v;
+    order[0] = 0;
+    queue[0] = 0;
+  }
+}
+
+// Copies the window's status and output size in bytes (out_len units of `unit` bytes: 1 for
+// the consensus, n_reads for the MSA's columns) to the caller's device words.
+__global__ void k_noalloc_finish(const WinResult* r, int64_t* d_res, int unit) {
+  if (threadIdx.x == 0) {
+    d_res[0] = r->status;
+    d_res[1] = r->status == MPOA_OK ? (int64_t)r->out_len * unit : 0;
+  }
+}
+
 }  // namespace mpoa
+
+int mpoa_scratch_address(int device, void** out) {
+  // one lookup per device, with that device current (the caller sets it); later calls reuse it
+  static std::mutex mu;
+  static void* addr[64] = {};
+  if (device < 0 || device >= 64 || !out) return -1;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(mpoa::scratch)) != hipSuccess || !p) return -1;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  return 0;
+}
+
+int mpoa_launch_noalloc(const mpoa::KernelArgs& a, const mpoa::WinDesc& wd, int nt, int cpl,
+                        int64_t* d_res, hipStream_t stream) {
+  const int unit = a.scores.result == 1 ? wd.n_reads : 1;
+  using namespace mpoa;
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(64), 0, stream, const_cast<WinDesc*>(a.win),
+                     wd, const_cast<int*>(a.order), a.queue);
+  if (hipGetLastError() != hipSuccess) return -1;
+  const int rc = mpoa_launch_windows(a, 1, nt, cpl, stream);
+  if (rc != 0) return rc;
+  hipLaunchKernelGGL(k_noalloc_finish, dim3(1), dim3(64), 0, stream, a.result, d_res, unit);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 #define MPOA_TILES(X) \
   X(64, 1) X(64, 2) X(64, 4) X(64, 8) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4) X(512, 2)
