@@ -87,6 +87,85 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
                      int64_t out_cap, int64_t* out_len, mpoa_call_stat* stats,
                      int32_t stats_cap, int32_t* n_stats);
 
+/* ------------------------------------------------------------------------------------
+ * Call-level API: minipoa's own functions (include/align.h:15-16, include/graph.h:16-30) over a
+ * graph that stays in device memory between calls, for callers that drive minipoa's per-read
+ * loop themselves (src/main.cpp:110-126):
+ *     for each read i:  res = alignment(...)  ->  graph::add_path(m, i, res, 1)
+ *                       ->  graph::topsort(para, i + 1 == n ? 1 : 0)
+ *     then graph::output_consensus / output_rc_msa / output_gfa.
+ * include/minipoa_amd.hpp wraps these in minipoa's C++ names. Calls on one graph are serial.
+ * ------------------------------------------------------------------------------------ */
+
+/* res_t (include/result.h:3-13): from >= 0 an existing node; from = -1 and aligned_id >= 0 a
+ * new node aligned to that column; both -1 a new node. base is the code 0..4 (A C G T N). */
+typedef struct mpoa_res {
+  int32_t from, base, aligned_id;
+} mpoa_res;
+
+typedef struct mpoa_graph mpoa_graph;
+
+/* graph::init (src/graph.cpp:129-135, called at src/main.cpp:99): a graph of the source
+ * (id 0) and the sink (id 1) on the context's device. params is the para_t of every later call
+ * on the graph (scores, band, -S, -r). */
+int mpoa_graph_create(mpoa_ctx* ctx, const mpoa_params* params, mpoa_graph** out);
+void mpoa_graph_destroy(mpoa_graph* g);
+
+/* poa(para, DAG, beg_id, end_id, qid, qseq, qlen, mpool, ab_band) (include/align.h:15,
+ * src/align.cpp:1353-1737): banded DP of qseq (qlen ASCII bases or codes 0..4) against the
+ * ranks [rank(beg_id), rank(end_id)] of the last topological order, then the backtrack.
+ * res receives the alignment in forward order (qlen + 1 entries; the first is the sentinel
+ * column's match to beg_id), *n_res its length (also when MPOA_E_OUT_CAP is returned), *score
+ * H[n-1][m] (minipoa's "score:" under -V 1). The graph must have been sorted (op 0) since its
+ * last add_path. Errors: MPOA_E_BAND and MPOA_E_BACKTRACK where minipoa exits (align.cpp:1584,
+ * :1730), MPOA_E_EMPTY_WINDOW for rank(end_id) < rank(beg_id). */
+int mpoa_poa(mpoa_graph* g, int32_t beg_id, int32_t end_id, const char* qseq, int32_t qlen,
+             int32_t ab_band, mpoa_res* res, int64_t cap, int64_t* n_res, int32_t* score);
+
+/* minimizer_t (include/minimizer.h:9-32) over the read set, built as src/main.cpp:84-99 does;
+ * needed by mpoa_alignment when params->seeding is set. Host only. */
+typedef struct mpoa_seeder mpoa_seeder;
+int mpoa_seeder_create(const mpoa_params* params, int32_t n_reads, const char* seq,
+                       const int64_t* seq_off, mpoa_seeder** out);
+void mpoa_seeder_destroy(mpoa_seeder* s);
+
+/* alignment(para, DAG, mm, rid, qseq, qlen, mpool) (include/align.h:16,
+ * src/align.cpp:1741-1828): the trivial path of a graph without bases, the whole-graph poa(),
+ * or with -S the read split at its anchors on the consensus, sub-window poa() calls and the
+ * anchor bases spliced between them (the graph is sorted first when it is not, as minipoa
+ * does). mm may be NULL without seeding. res / cap / n_res as for mpoa_poa. */
+int mpoa_alignment(mpoa_graph* g, mpoa_seeder* mm, int32_t rid, const char* qseq, int32_t qlen,
+                   mpoa_res* res, int64_t cap, int64_t* n_res);
+
+/* graph::add_path(m, seq_id, res, 1) (src/graph.cpp:170-235, called at src/main.cpp:118): new
+ * nodes and edges of one read's alignment; seq_id is the read's index, and reads are added in
+ * order 0, 1, 2, ... (the MSA and GFA rows follow it). Entries are checked on the host (node
+ * ids below the node count, bases 0..4): MPOA_E_ARG otherwise. Marks the graph unsorted. */
+int mpoa_graph_add_path(mpoa_graph* g, int32_t seq_id, const mpoa_res* res, int64_t n_res);
+
+/* graph::topsort(para, op) (src/graph.cpp:21-128, called at src/main.cpp:120): op 0, the
+ * LIFO topological order with hlen/tlen (and, with -S, build_consensus); op 1, the order of
+ * aligned groups (MSA columns). */
+int mpoa_graph_topsort(mpoa_graph* g, int32_t op);
+
+/* graph::output_consensus (result 0), output_rc_msa (1) or output_gfa (2)
+ * (src/graph.cpp:237-433, called at src/main.cpp:124-126) over the reads added so far, as the
+ * text minipoa prints; names / comments as for mpoa_align_reads. The MSA of a graph not sorted
+ * since its last add_path is taken after an op 1 sort, as output_rc_msa does. *out_len
+ * receives the size (also with MPOA_E_OUT_CAP); out NULL sizes only. */
+int mpoa_graph_output(mpoa_graph* g, int32_t result, const char* name, const int64_t* name_off,
+                      const char* comment, const int64_t* comment_off, char* out, int64_t out_cap,
+                      int64_t* out_len);
+
+/* The graph as host arrays (node_t fields, include/node.h:5-68): node count and in-edge count,
+ * then per node id its base, par_id and aligned_node[5]; in- and out-edge CSR in list order
+ * with weights; and by rank the node id, hlen and tlen of the last op 0 sort (rank_to_id may
+ * be shorter after op 1). Pass NULL arrays to size. */
+int mpoa_graph_export(mpoa_graph* g, int32_t* n_nodes, int32_t* n_edges, uint8_t* base,
+                      int32_t* in_off, int32_t* in_src, int32_t* in_w, int32_t* out_off,
+                      int32_t* out_dst, int32_t* out_w, int32_t* rank_to_id, int32_t* hlen,
+                      int32_t* tlen, int32_t* par_id, int32_t* aligned);
+
 /* Allocation-free form of mpoa_align_reads. No counterpart in minipoa: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
  * forbid hipMalloc). One window, no seeding (params->seeding must be 0), consensus (-r 0) or

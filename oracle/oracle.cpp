@@ -1376,6 +1376,13 @@ extern "C" int orc_batch(const orc_params* p, int32_t n_windows, const int32_t* 
   return ORC_OK;
 }
 
+namespace orc {
+int export_graph(const Graph& G, int32_t* n_nodes, int32_t* n_edges, uint8_t* base,
+                 int32_t* in_off, int32_t* in_src, int32_t* in_w, int32_t* out_off,
+                 int32_t* out_dst, int32_t* out_w, int32_t* rank_to_id, int32_t* hlen,
+                 int32_t* tlen, int32_t* par_id, int32_t* aligned);
+}  // namespace orc
+
 extern "C" int orc_graph(const orc_params* p, int32_t n_reads, const char* seq,
                          const int64_t* seq_off, int32_t* n_nodes, int32_t* n_edges,
                          uint8_t* base, int32_t* in_off, int32_t* in_src, int32_t* in_w,
@@ -1387,7 +1394,15 @@ extern "C" int orc_graph(const orc_params* p, int32_t n_reads, const char* seq,
   WindowRun W;
   int rc = run_window(P, n_reads, seq, seq_off, {}, {}, nullptr, W, n_reads);
   if (rc != ORC_OK) return rc;
-  const Graph& G = W.G;
+  return orc::export_graph(W.G, n_nodes, n_edges, base, in_off, in_src, in_w, out_off, out_dst,
+                           out_w, rank_to_id, hlen, tlen, par_id, aligned);
+}
+
+namespace orc {
+int export_graph(const Graph& G, int32_t* n_nodes, int32_t* n_edges, uint8_t* base,
+                 int32_t* in_off, int32_t* in_src, int32_t* in_w, int32_t* out_off,
+                 int32_t* out_dst, int32_t* out_w, int32_t* rank_to_id, int32_t* hlen,
+                 int32_t* tlen, int32_t* par_id, int32_t* aligned) {
   const int N = (int)G.node.size();
   if (n_nodes) *n_nodes = N;
   if (n_edges) *n_edges = G.n_edges();
@@ -1419,6 +1434,7 @@ extern "C" int orc_graph(const orc_params* p, int32_t n_reads, const char* seq,
   }
   return ORC_OK;
 }
+}  // namespace orc
 
 extern "C" int orc_dp_matrix(const orc_params* p, int32_t n_reads, const char* seq,
                              const int64_t* seq_off, int32_t* n_rows, int64_t* total,
@@ -1474,3 +1490,117 @@ extern "C" int orc_anchors(const orc_params* p, int32_t n_reads, const char* seq
   return ORC_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Call-level mirror of minipoa's functions on one graph (graph::init, poa(), alignment(),
+// graph::add_path, graph::topsort, the outputs), the checker of the library's call-level API.
+// ---------------------------------------------------------------------------------------
+struct orc_dag {
+  orc::Params P;
+  orc::Graph G;
+  orc::Seeder S;
+  bool seeded = false;
+  std::vector<int> read_len;  // reads added so far (add_path), in order
+};
+
+static void put_res(const std::vector<orc::Res>& res, int32_t* out, int64_t cap, int64_t* n) {
+  if (n) *n = (int64_t)res.size();
+  if (!out) return;
+  for (int64_t i = 0; i < (int64_t)res.size() && i < cap; i++) {
+    out[3 * i] = res[i].from;
+    out[3 * i + 1] = res[i].base;
+    out[3 * i + 2] = res[i].aligned_id;
+  }
+}
+
+extern "C" orc_dag* orc_dag_new(const orc_params* p, int32_t n_reads, const char* seq,
+                                    const int64_t* seq_off) {
+  if (!p) return nullptr;
+  orc_dag* g = new orc_dag();
+  g->P = orc::normalise(*p);
+  g->G.init();
+  if (g->P.seeding && n_reads > 0 && seq && seq_off) {
+    g->S.init(g->P, n_reads, seq, seq_off);
+    g->seeded = true;
+  }
+  return g;
+}
+
+extern "C" void orc_dag_free(orc_dag* g) { delete g; }
+
+extern "C" int orc_dag_poa(orc_dag* g, int32_t beg_id, int32_t end_id, const char* q,
+                             int32_t qlen, int32_t ab_band, int32_t* res, int64_t cap,
+                             int64_t* n_res, int32_t* score) {
+  if (!g || qlen < 0 || beg_id < 0 || end_id < 0 || beg_id >= (int)g->G.node.size() ||
+      end_id >= (int)g->G.node.size())
+    return ORC_E_ARG;
+  std::vector<uint8_t> t(qlen);
+  for (int j = 0; j < qlen; j++) t[j] = orc::code_of((uint8_t)q[j]);
+  std::vector<orc::Res> out;
+  orc::CallOut co;
+  const int rc = orc::poa(g->P, g->G, beg_id, end_id, t.data(), qlen, ab_band != 0, out, &co,
+                          nullptr);
+  if (score) *score = co.score;
+  put_res(out, res, cap, n_res);
+  return rc;
+}
+
+extern "C" int orc_dag_alignment(orc_dag* g, int32_t rid, const char* q, int32_t qlen,
+                                   int32_t* res, int64_t cap, int64_t* n_res) {
+  if (!g || qlen < 0 || (g->P.seeding && !g->seeded)) return ORC_E_ARG;
+  std::vector<orc::Res> out;
+  const int rc = orc::alignment(g->P, g->G, &g->S, rid, q, qlen, out, nullptr);
+  put_res(out, res, cap, n_res);
+  return rc;
+}
+
+extern "C" int orc_dag_add_path(orc_dag* g, int32_t seq_id, const int32_t* res,
+                                  int64_t n_res) {
+  if (!g || !res || n_res < 1) return ORC_E_ARG;
+  std::vector<orc::Res> r((size_t)n_res);
+  for (int64_t i = 0; i < n_res; i++)
+    r[i] = {res[3 * i], (uint8_t)res[3 * i + 1], res[3 * i + 2]};
+  g->G.add_path(seq_id, r, 1);
+  if (seq_id == (int)g->read_len.size()) g->read_len.push_back((int)n_res - 1);
+  return ORC_OK;
+}
+
+extern "C" int orc_dag_topsort(orc_dag* g, int32_t op) {
+  if (!g || (op != 0 && op != 1)) return ORC_E_ARG;
+  g->G.topsort(g->P, op);
+  return ORC_OK;
+}
+
+extern "C" int orc_dag_output(orc_dag* g, int32_t result, const char* name,
+                                const int64_t* name_off, const char* comment,
+                                const int64_t* comment_off, char* out, int64_t out_cap,
+                                int64_t* out_len) {
+  if (!g || result < 0 || result > 2) return ORC_E_ARG;
+  const int n = (int)g->read_len.size();
+  const std::vector<std::string> nm = orc::split(name, name_off, n);
+  const std::vector<std::string> cm = orc::split(comment, comment_off, n);
+  orc::Text o;
+  if (result == 0) {
+    g->G.build_consensus();
+    o.put(">consensus sequence\n");
+    o.put(g->G.cons);
+    o.put('\n');
+  } else if (result == 1) {
+    orc::output_msa(g->G, g->P, n, nm, cm, o);
+  } else {
+    orc::output_gfa(g->G, n, g->read_len, nm, o);
+  }
+  if (out_len) *out_len = (int64_t)o.s.size();
+  if (!out || (int64_t)o.s.size() > out_cap) return out ? ORC_E_OUT_CAP : ORC_OK;
+  std::memcpy(out, o.s.data(), o.s.size());
+  return ORC_OK;
+}
+
+extern "C" int orc_dag_export(orc_dag* g, int32_t* n_nodes, int32_t* n_edges, uint8_t* base,
+                                int32_t* in_off, int32_t* in_src, int32_t* in_w,
+                                int32_t* out_off, int32_t* out_dst, int32_t* out_w,
+                                int32_t* rank_to_id, int32_t* hlen, int32_t* tlen,
+                                int32_t* par_id, int32_t* aligned) {
+  if (!g) return ORC_E_ARG;
+  return orc::export_graph(g->G, n_nodes, n_edges, base, in_off, in_src, in_w, out_off, out_dst,
+                           out_w, rank_to_id, hlen, tlen, par_id, aligned);
+}

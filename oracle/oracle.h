@@ -104,6 +104,32 @@ int orc_anchors(const orc_params* p, int32_t n_reads, const char* seq, const int
                 int32_t rid, const char* cons, int32_t cons_len, uint64_t* out, int64_t cap,
                 int64_t* n);
 
+/* Call-level mirror of minipoa's functions on one graph, the checker of the library's
+ * call-level API. orc_dag_new: graph::init (src/graph.cpp:129-135) and, with seeding, the
+ * minimizer_t of the read set (src/minimizer.cpp); res arrays hold 3 int32 per res_t entry
+ * (from, base code, aligned_id; include/result.h:3-13). orc_dag_poa: poa()
+ * (src/align.cpp:1353-1737); orc_dag_alignment: alignment() (:1741-1828); orc_dag_add_path:
+ * graph::add_path(m, seq_id, res, 1) (src/graph.cpp:170-235); orc_dag_topsort:
+ * graph::topsort (:21-128); orc_dag_output: the -r 0/1/2 text (:237-433) over the reads
+ * added so far; orc_dag_export: the graph as orc_graph exports it. */
+typedef struct orc_dag orc_dag;
+orc_dag* orc_dag_new(const orc_params* p, int32_t n_reads, const char* seq,
+                         const int64_t* seq_off);
+void orc_dag_free(orc_dag* g);
+int orc_dag_poa(orc_dag* g, int32_t beg_id, int32_t end_id, const char* q, int32_t qlen,
+                  int32_t ab_band, int32_t* res, int64_t cap, int64_t* n_res, int32_t* score);
+int orc_dag_alignment(orc_dag* g, int32_t rid, const char* q, int32_t qlen, int32_t* res,
+                        int64_t cap, int64_t* n_res);
+int orc_dag_add_path(orc_dag* g, int32_t seq_id, const int32_t* res, int64_t n_res);
+int orc_dag_topsort(orc_dag* g, int32_t op);
+int orc_dag_output(orc_dag* g, int32_t result, const char* name, const int64_t* name_off,
+                     const char* comment, const int64_t* comment_off, char* out, int64_t out_cap,
+                     int64_t* out_len);
+int orc_dag_export(orc_dag* g, int32_t* n_nodes, int32_t* n_edges, uint8_t* base,
+                     int32_t* in_off, int32_t* in_src, int32_t* in_w, int32_t* out_off,
+                     int32_t* out_dst, int32_t* out_w, int32_t* rank_to_id, int32_t* hlen,
+                     int32_t* tlen, int32_t* par_id, int32_t* aligned);
+
 #ifdef __cplusplus
 }
 #endif

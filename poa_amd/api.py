@@ -29,7 +29,9 @@ EXPORTS = [
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
     "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph", "mpoa_batch_set_forms",
-    "mpoa_set_forms", "mpoa_align_reads_noalloc",
+    "mpoa_set_forms", "mpoa_align_reads_noalloc", "mpoa_graph_create", "mpoa_graph_destroy",
+    "mpoa_poa", "mpoa_seeder_create", "mpoa_seeder_destroy", "mpoa_alignment",
+    "mpoa_graph_add_path", "mpoa_graph_topsort", "mpoa_graph_output", "mpoa_graph_export",
     "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
@@ -125,6 +127,18 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
+        L.mpoa_graph_create.argtypes = [VP, P(Params), P(VP)]
+        L.mpoa_graph_destroy.argtypes = [VP]
+        L.mpoa_graph_destroy.restype = None
+        L.mpoa_poa.argtypes = [VP, I32, I32, VP, I32, I32, VP, I64, P(I64), P(I32)]
+        L.mpoa_seeder_create.argtypes = [P(Params), I32, VP, VP, P(VP)]
+        L.mpoa_seeder_destroy.argtypes = [VP]
+        L.mpoa_seeder_destroy.restype = None
+        L.mpoa_alignment.argtypes = [VP, VP, I32, VP, I32, VP, I64, P(I64)]
+        L.mpoa_graph_add_path.argtypes = [VP, I32, VP, I64]
+        L.mpoa_graph_topsort.argtypes = [VP, I32]
+        L.mpoa_graph_output.argtypes = [VP, I32, VP, VP, VP, VP, VP, I64, P(I64)]
+        L.mpoa_graph_export.argtypes = [VP, P(I32), P(I32)] + [VP] * 12
         _lib = L
     return _lib
 
@@ -323,3 +337,124 @@ class Batch:
             self.free()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------------------
+# Call-level API: minipoa's poa() / alignment() / graph::add_path / graph::topsort / outputs
+# on a graph kept in device memory (include/mpoa.h, "Call-level API").
+# ---------------------------------------------------------------------------------------
+RES_DTYPE = np.dtype([("from", np.int32), ("base", np.int32), ("aligned_id", np.int32)])
+
+
+class Seeder:
+    """minipoa's minimizer_t over a read set (needed by Graph.alignment with -S)."""
+
+    def __init__(self, params: Params, seq: np.ndarray, off: np.ndarray):
+        seq = np.ascontiguousarray(seq, np.uint8)
+        off = np.ascontiguousarray(off, np.int64)
+        h = ctypes.c_void_p()
+        _check(lib().mpoa_seeder_create(ctypes.byref(params), len(off) - 1, _ptr(seq), _ptr(off),
+                                        ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().mpoa_seeder_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A POA graph in device memory driven call by call, as minipoa's main loop drives its
+    graph (src/main.cpp:110-126). res arrays are structured arrays of RES_DTYPE."""
+
+    def __init__(self, ctx: Context, params: Params):
+        self.ctx = ctx
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib().mpoa_graph_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().mpoa_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _res_call(fn, *args) -> np.ndarray:
+        n = ctypes.c_int64()
+        _check(fn(*args, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), RES_DTYPE)
+        _check(fn(*args, _ptr(out), len(out), ctypes.byref(n)))
+        return out[:n.value]
+
+    def poa(self, beg_id: int, end_id: int, q: bytes, ab_band: bool):
+        """poa() on the rank window [rank(beg_id), rank(end_id)]: (res, score)."""
+        score = ctypes.c_int32()
+        n = ctypes.c_int64()
+        buf = ctypes.create_string_buffer(q, len(q) + 1)
+        out = np.zeros(len(q) + 2, RES_DTYPE)
+        _check(lib().mpoa_poa(self.h, beg_id, end_id, buf, len(q), int(ab_band), _ptr(out),
+                              len(out), ctypes.byref(n), ctypes.byref(score)))
+        return out[:n.value], score.value
+
+    def alignment(self, rid: int, q: bytes, seeder: Seeder | None = None) -> np.ndarray:
+        """alignment(): the res of read rid against the graph."""
+        buf = ctypes.create_string_buffer(q, len(q) + 1)
+        return self._res_call(lib().mpoa_alignment, self.h, seeder.h if seeder else None, rid,
+                              buf, len(q))
+
+    def add_path(self, seq_id: int, res: np.ndarray) -> None:
+        """graph::add_path of a res given as RES_DTYPE records or as (n, 3) int32 rows."""
+        if res.dtype != RES_DTYPE:
+            res = np.ascontiguousarray(res, np.int32).reshape(-1, 3).view(RES_DTYPE).reshape(-1)
+        res = np.ascontiguousarray(res, RES_DTYPE)
+        _check(lib().mpoa_graph_add_path(self.h, seq_id, _ptr(res), len(res)))
+
+    def topsort(self, op: int) -> None:
+        _check(lib().mpoa_graph_topsort(self.h, op))
+
+    def output(self, result: int, names: list[str] | None = None,
+               comments: list[str] | None = None) -> str:
+        nb, no = _names(names)
+        cb, co = _names(comments)
+        n = ctypes.c_int64()
+        _check(lib().mpoa_graph_output(self.h, result, _ptr(nb), _ptr(no), _ptr(cb), _ptr(co),
+                                       None, 0, ctypes.byref(n)))
+        buf = np.zeros(max(n.value, 1), np.uint8)
+        _check(lib().mpoa_graph_output(self.h, result, _ptr(nb), _ptr(no), _ptr(cb), _ptr(co),
+                                       _ptr(buf), len(buf), ctypes.byref(n)))
+        return buf[:n.value].tobytes().decode()
+
+    def export(self) -> dict:
+        """Node bases, par_id, aligned_node, in/out edge CSR with weights, rank -> id, hlen,
+        tlen (by rank of the last op 0 sort)."""
+        nn, ne = ctypes.c_int32(), ctypes.c_int32()
+        L = lib()
+        _check(L.mpoa_graph_export(self.h, ctypes.byref(nn), ctypes.byref(ne), *([None] * 12)))
+        N, E = nn.value, ne.value
+        d = {"base": np.zeros(N, np.uint8), "in_off": np.zeros(N + 1, np.int32),
+             "in_src": np.zeros(max(E, 1), np.int32), "in_w": np.zeros(max(E, 1), np.int32),
+             "out_off": np.zeros(N + 1, np.int32), "out_dst": np.zeros(max(E, 1), np.int32),
+             "out_w": np.zeros(max(E, 1), np.int32), "rank_to_id": np.zeros(N, np.int32),
+             "hlen": np.zeros(N, np.int32), "tlen": np.zeros(N, np.int32),
+             "par_id": np.zeros(N, np.int32), "aligned": np.zeros(5 * N, np.int32)}
+        keys = ("base", "in_off", "in_src", "in_w", "out_off", "out_dst", "out_w", "rank_to_id",
+                "hlen", "tlen", "par_id", "aligned")
+        _check(L.mpoa_graph_export(self.h, ctypes.byref(nn), ctypes.byref(ne),
+                                   *[_ptr(d[k]) for k in keys]))
+        d["in_src"], d["in_w"] = d["in_src"][:E], d["in_w"][:E]
+        d["out_dst"], d["out_w"] = d["out_dst"][:E], d["out_w"][:E]
+        return d
+

@@ -3,6 +3,7 @@
 * ``poa_amd/libmpoa.so``        the product: HIP kernels for gfx950 + the C ABI (include/mpoa.h)
 * ``poa_amd/libmpoa_synth.so``  the seeded workload generator (include/mpoa_synth.h)
 * ``poa_amd/minipoa_amd``       the command-line tool with minipoa's flags
+* ``poa_amd/minipoa_loop``      minipoa's main loop written against the C++ facade
 * ``oracle/``                   the CPU checker (oracle/Makefile), and, where the upstream
                                 sources are present, the upstream binary in oracle/_ref/
 """
@@ -30,7 +31,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "sched.cpp", "seed.cpp"]
+PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "graph_api.cpp", "sched.cpp", "seed.cpp"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:
@@ -71,6 +72,14 @@ def build_cli() -> str | None:
     return out
 
 
+def build_loop() -> str:
+    """minipoa's main loop over the C++ facade (include/minipoa_amd.hpp)."""
+    out = os.path.join(PKG, "minipoa_loop")
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", "-o", out,
+          os.path.join(CSRC, "minipoa_loop.cpp"), f"-L{PKG}", "-lmpoa", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"])
     if os.path.isdir(os.path.join(REF_DIR, "src")):
@@ -83,11 +92,12 @@ def build_all() -> None:
     build_synth()
     build_product()
     build_cli()
+    build_loop()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "synth": build_synth, "product": build_product, "cli": build_cli,
+        {"all": build_all, "synth": build_synth, "product": build_product, "cli": build_cli, "loop": build_loop,
          "oracle": build_oracle}[w]()

@@ -1,0 +1,88 @@
+// minipoa's main loop (src/main.cpp:84-126) written against the C++ facade
+// include/minipoa_amd.hpp: the program a minipoa user keeps, with the alignment path on the
+// GPU. Reads one plain FASTA file; flags -M -X -O -E -b -f -B -S -k -w -r as minipoa's
+// (src/main.cpp:14-35). Prints what minipoa prints.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "minipoa_amd.hpp"
+
+using namespace minipoa_amd;
+
+static std::vector<seq_t> read_fasta(const char* path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error(std::string("cannot open ") + path);
+  std::vector<seq_t> seqs;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.empty()) continue;
+    if (line[0] == '>') {
+      seq_t s;
+      const size_t sp = line.find_first_of(" \t");
+      s.name = line.substr(1, sp == std::string::npos ? std::string::npos : sp - 1);
+      if (sp != std::string::npos) s.comment = line.substr(sp + 1);
+      seqs.push_back(s);
+    } else if (!seqs.empty()) {
+      seqs.back().seq += line;
+    }
+  }
+  return seqs;
+}
+
+int main(int argc, char** argv) {
+  para_t* para = new para_t();
+  const char* path = nullptr;
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto val = [&]() { return i + 1 < argc ? std::atoi(argv[++i]) : 0; };
+    if (a == "-M") para->match = val();
+    else if (a == "-X") para->mismatch = val();
+    else if (a == "-O") para->gap_open1 = val();
+    else if (a == "-E") para->gap_ext1 = val();
+    else if (a == "-b") para->b = val();
+    else if (a == "-f") para->f = val();
+    else if (a == "-B") para->ab_band = true;
+    else if (a == "-S") para->enable_seeding = true;
+    else if (a == "-k") para->k = val();
+    else if (a == "-w") para->mm_w = val();
+    else if (a == "-r") para->result = val();
+    else path = argv[i];
+  }
+  if (!path) {
+    std::cerr << "usage: minipoa_loop [flags] reads.fa\n";
+    return 1;
+  }
+  try {
+    std::vector<seq_t> seqs = read_fasta(path);
+    graph* DAG = new graph();
+    DAG->init(para);
+    minimizer_t* mm = new minimizer_t(para, seqs);
+    for (int i = 0; i < (int)seqs.size(); i++) {
+      const int rid = mm->ord[i];
+      std::vector<res_t> res =
+          alignment(para, DAG, mm, rid, seqs[rid].seq.c_str(), (int)seqs[rid].seq.size());
+      DAG->add_path(para->m, i, res, 1);
+      DAG->topsort(para, i + 1 == (int)seqs.size());
+    }
+    if (para->result == 0) DAG->output_consensus();
+    else if (para->result == 1) DAG->output_rc_msa(para, mm->rid_to_ord, seqs);
+    else if (para->result == 2) DAG->output_gfa(mm->rid_to_ord, seqs);
+    delete mm;
+    delete DAG;
+  } catch (const error& e) {
+    // minipoa exits 0 on a band violation (align.cpp:1584-1588) and 1 on a backtrack error
+    std::cerr << e.what() << "\n";
+    return e.code == MPOA_E_BAND ? 0 : 1;
+  } catch (const std::exception& e) {
+    std::cerr << e.what() << "\n";
+    return 1;
+  }
+  delete para;
+  return 0;
+}

@@ -14,6 +14,7 @@
 
 #include "mpoa.h"
 #include "pipeline.h"
+#include "host.h"
 #include "seed.h"
 
 using mpoa::CallStat;
@@ -28,126 +29,16 @@ using mpoa::SegResult;
 using mpoa::WinState;
 namespace seed = mpoa::seed;
 
-struct mpoa_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;   // small-window launches, copies
-  hipStream_t stream2 = nullptr;  // medium-tile launches, concurrent with the first
-  hipStream_t stream3 = nullptr;  // wide-tile launches, concurrent with both
-  int n_cu = 0;
-  size_t mem_total = 0;
-  int wall_khz = 100000;          // device wall clock (the k_hold timer)
-  // mpoa_align_reads: the last call whose text did not fit its buffer, for the follow-up call
-  // with the same arguments
-  std::string held_key, held_text;
-  std::vector<mpoa_call_stat> held_calls;
-  int forms = 0;  // MPOA_FORM_* test hook: the forms of batches created afterwards
-  uint8_t* scratch = nullptr;  // this device's allocation-free scratch array (device address)
-};
+namespace mpoa {
+namespace host {
+thread_local std::string g_err;
+}  // namespace host
+}  // namespace mpoa
 
 namespace {
 
-thread_local std::string g_err;
+using namespace mpoa::host;
 
-#define HIPCHK(expr)                                                                      \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess) {                                                              \
-      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
-      return e_ == hipErrorOutOfMemory ? MPOA_E_OOM : MPOA_E_DEVICE;                      \
-    }                                                                                    \
-  } while (0)
-
-// nt4 encoding (src/sequence.cpp:6-23): A C G T/U -> 0..3, anything else 4.
-inline uint8_t code_of_char(unsigned char c) {
-  switch (c) {
-    case 'A': case 'a': return 0;
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': case 'U': case 'u': return 3;
-    default: return 4;
-  }
-}
-
-// initPara (src/handle_input.cpp:45-77) + o1 = O + E (src/align.cpp:1369).
-DpScores scores_of(const mpoa_params& p) {
-  DpScores s;
-  s.match = p.match;
-  s.mismatch = p.mismatch > 0 ? -p.mismatch : p.mismatch;
-  const int O = p.gap_open > 0 ? -p.gap_open : p.gap_open;
-  s.e1 = p.gap_ext > 0 ? -p.gap_ext : p.gap_ext;
-  s.o1 = O + s.e1;
-  s.b = p.band_b;
-  s.f = p.band_f;
-  s.ab_band = p.ab_band;
-  s.result = p.result;
-  return s;
-}
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  int alloc(size_t count) {
-    release();
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
-      return MPOA_E_OOM;
-    }
-    n = count;
-    return MPOA_OK;
-  }
-};
-
-struct WinInfo {
-  int n_reads;
-  int qmax;
-  int64_t qtot;
-};
-
-// Capacities estimated from the reads alone. Nodes: the longest read plus 0.16 node per other
-// base (reads at up to 15% error add at most ~0.12 new nodes per base); DP cells: nodes times
-// the static band width 2w + 64. The device checks every capacity exactly and a window that
-// outgrows one is re-run with worst-case capacities.
-struct Estimate {
-  int nodes, edges;
-  int64_t dp;
-};
-
-Estimate estimate(const WinInfo& w, const DpScores& s, bool full_width) {
-  Estimate e;
-  int64_t n = 2 + w.qmax + (int64_t)std::ceil(0.16 * (double)(w.qtot - w.qmax));
-  n = std::min<int64_t>(n, 2 + w.qtot);
-  e.nodes = (int)n;
-  e.edges = (int)std::min<int64_t>(3 * n + w.n_reads, w.qtot + w.n_reads + 2);
-  const int m = w.qmax + 1;
-  int64_t width;
-  if (s.f > 0 && !full_width) {
-    const int wb = s.b + m / s.f;
-    width = std::min<int64_t>(2LL * wb + 64, (int64_t)(m / 8 + 2) * 8);
-  } else {
-    width = (int64_t)(m / 8 + 2) * 8;
-  }
-  e.dp = n * width;
-  return e;
-}
-
-// Exact worst case: every base a new node; every row full width.
-Estimate worst_case(const WinInfo& w) {
-  Estimate e;
-  e.nodes = (int)std::min<int64_t>(2 + w.qtot, INT32_MAX / 64);
-  e.edges = (int)std::min<int64_t>(w.qtot + w.n_reads + 2, INT32_MAX / 64);
-  e.dp = (int64_t)e.nodes * ((w.qmax + 1) / 8 + 2) * 8;
-  return e;
-}
 
 mpoa_call_stat call_stat_of(const CallStat& c, int read, int window) {
   mpoa_call_stat o{};
@@ -901,6 +792,72 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
 
 }  // namespace
 
+namespace mpoa {
+namespace host {
+
+int window_text(int result, const WinResult& r, const uint8_t* d_out, const int64_t* d_gfa,
+                int n_reads, const char* name, const int64_t* name_off, const char* comment,
+                const int64_t* comment_off, std::string* out) {
+  std::string& text = *out;
+  text.clear();
+  if (result == 0) {
+    std::vector<char> cons((size_t)std::max(r.out_len, 1));
+    if (r.out_len > 0 &&
+        hipMemcpy(cons.data(), d_out, (size_t)r.out_len, hipMemcpyDeviceToHost) != hipSuccess) {
+      return MPOA_E_DEVICE;
+    }
+    text = ">consensus sequence\n";
+    text.append(cons.data(), (size_t)r.out_len);
+    text.push_back('\n');
+  } else if (result == 2) {
+    // output_gfa (src/graph.cpp:358-433): the device names path lines by read index; the
+    // read names replace those where given.
+    std::vector<char> g((size_t)std::max(r.out_len, 1));
+    std::vector<int64_t> at((size_t)n_reads + 1);
+    if (r.out_len > 0 &&
+        (hipMemcpy(g.data(), d_out, (size_t)r.out_len, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(at.data(), d_gfa, at.size() * sizeof(int64_t), hipMemcpyDeviceToHost) !=
+             hipSuccess)) {
+      return MPOA_E_DEVICE;
+    }
+    if (r.out_len > 0) {
+      text.assign(g.data(), (size_t)at[0]);
+      for (int i = 0; i < n_reads; i++) {
+        const char* line = g.data() + at[i];
+        const int64_t len = at[i + 1] - at[i];
+        const char* tab = (const char*)std::memchr(line + 2, '\t', (size_t)(len - 2));
+        std::string nm;
+        if (name && name_off) nm.assign(name + name_off[i], name + name_off[i + 1]);
+        text += "P\t";
+        text += nm.empty() ? std::to_string(i + 1) : nm;
+        if (tab) text.append(tab, (size_t)(line + len - tab));
+      }
+    }
+  } else {
+    // output_rc_msa (src/graph.cpp:237-251)
+    const int L = r.out_len;
+    std::vector<char> rows((size_t)std::max<int64_t>((int64_t)n_reads * L, 1));
+    if ((int64_t)n_reads * L > 0 &&
+        hipMemcpy(rows.data(), d_out, (size_t)n_reads * L, hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+      return MPOA_E_DEVICE;
+    }
+    for (int i = 0; i < n_reads; i++) {
+      std::string nm, cm;
+      if (name && name_off) nm.assign(name + name_off[i], name + name_off[i + 1]);
+      if (comment && comment_off) cm.assign(comment + comment_off[i], comment + comment_off[i + 1]);
+      if (nm.empty()) text += ">" + std::to_string(i + 1) + "\n";
+      else text += ">" + nm + " " + cm + "\n";
+      text.append(rows.data() + (int64_t)i * L, (size_t)L);
+      text.push_back('\n');
+    }
+  }
+  return MPOA_OK;
+}
+
+}  // namespace host
+}  // namespace mpoa
+
 extern "C" {
 
 const char* mpoa_strerror(int rc) {
@@ -1493,60 +1450,11 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   }
   const WinResult r = b->res.empty() ? WinResult{} : b->res[0];
   std::string text;
-  if (params->result == 0) {
-    std::vector<char> cons((size_t)std::max(r.out_len, 1));
-    if (r.out_len > 0 &&
-        hipMemcpy(cons.data(), b->d_out.p, (size_t)r.out_len, hipMemcpyDeviceToHost) != hipSuccess) {
-      mpoa_batch_free(b);
-      return MPOA_E_DEVICE;
-    }
-    text = ">consensus sequence\n";
-    text.append(cons.data(), (size_t)r.out_len);
-    text.push_back('\n');
-  } else if (params->result == 2) {
-    // output_gfa (src/graph.cpp:358-433): the device names path lines by read index; the
-    // read names replace those where given.
-    std::vector<char> g((size_t)std::max(r.out_len, 1));
-    std::vector<int64_t> at((size_t)n_reads + 1);
-    if (r.out_len > 0 &&
-        (hipMemcpy(g.data(), b->d_out.p, (size_t)r.out_len, hipMemcpyDeviceToHost) != hipSuccess ||
-         hipMemcpy(at.data(), b->d_gfa.p, at.size() * sizeof(int64_t), hipMemcpyDeviceToHost) !=
-             hipSuccess)) {
-      mpoa_batch_free(b);
-      return MPOA_E_DEVICE;
-    }
-    if (r.out_len > 0) {
-      text.assign(g.data(), (size_t)at[0]);
-      for (int i = 0; i < n_reads; i++) {
-        const char* line = g.data() + at[i];
-        const int64_t len = at[i + 1] - at[i];
-        const char* tab = (const char*)std::memchr(line + 2, '\t', (size_t)(len - 2));
-        std::string nm;
-        if (name && name_off) nm.assign(name + name_off[i], name + name_off[i + 1]);
-        text += "P\t";
-        text += nm.empty() ? std::to_string(i + 1) : nm;
-        if (tab) text.append(tab, (size_t)(line + len - tab));
-      }
-    }
-  } else {
-    // output_rc_msa (src/graph.cpp:237-251)
-    const int L = r.out_len;
-    std::vector<char> rows((size_t)std::max<int64_t>((int64_t)n_reads * L, 1));
-    if ((int64_t)n_reads * L > 0 &&
-        hipMemcpy(rows.data(), b->d_out.p, (size_t)n_reads * L, hipMemcpyDeviceToHost) !=
-            hipSuccess) {
-      mpoa_batch_free(b);
-      return MPOA_E_DEVICE;
-    }
-    for (int i = 0; i < n_reads; i++) {
-      std::string nm, cm;
-      if (name && name_off) nm.assign(name + name_off[i], name + name_off[i + 1]);
-      if (comment && comment_off) cm.assign(comment + comment_off[i], comment + comment_off[i + 1]);
-      if (nm.empty()) text += ">" + std::to_string(i + 1) + "\n";
-      else text += ">" + nm + " " + cm + "\n";
-      text.append(rows.data() + (int64_t)i * L, (size_t)L);
-      text.push_back('\n');
-    }
+  rc = mpoa::host::window_text(params->result, r, b->d_out.p, b->d_gfa.p, n_reads, name,
+                               name_off, comment, comment_off, &text);
+  if (rc != MPOA_OK) {
+    mpoa_batch_free(b);
+    return rc;
   }
   mpoa_batch_free(b);
   return deliver(text, calls);

@@ -193,6 +193,31 @@ struct SeedArgs {
   int cons_cap;
 };
 
+// One step of minipoa's per-read loop on a graph kept in device memory between calls (the
+// call-level API: poa(), graph::add_path, graph::topsort, the outputs). The graph is slot 0 of
+// a's pools, its counters and status in `state`.
+enum GraphOp { GOP_INIT = 0, GOP_POA = 1, GOP_ADD_PATH = 2, GOP_TOPSORT = 3, GOP_OUTPUT = 4 };
+
+struct GraphOpOut {
+  int status, score, rows, qlen, width, pad;
+  int64_t cells, edge_cells;
+};
+
+struct GraphOpArgs {
+  KernelArgs a;           // a.win[0] / a.read_off: the reads added so far (outputs)
+  WinState* state;
+  GraphOpOut* out;
+  int op;
+  int beg_id, end_id, adaptive;  // GOP_POA: rank window of node ids, band mode
+  const uint8_t* q;              // GOP_POA: query (ASCII or codes 0..4), qlen bases
+  int qlen;
+  int seq_id, n_res, path0;      // GOP_ADD_PATH: read index, res entries (already in the
+                                 // graph's res arrays), bases of the reads before it
+  int sort_op, seeding;          // GOP_TOPSORT: 0 or 1; with seeding, the consensus too
+  int* cons_out;                 // (node id, rank, base) per consensus position
+  int cons_cap;
+};
+
 __host__ __device__ inline Arena carve(const KernelArgs& a, int slot) {
   return carve_pools(a.caps, a.int_pool, a.byte_pool, a.dp_pool, slot);
 }
@@ -227,3 +252,5 @@ int mpoa_scratch_address(int device, void** out);
 // tile.
 int mpoa_launch_noalloc(const mpoa::KernelArgs& a, const mpoa::WinDesc& wd, int nt, int cpl,
                         int64_t* d_res, hipStream_t stream);
+// One graph operation (GraphOpArgs::op) by one 512-thread workgroup. 0, or -1 on a launch error.
+int mpoa_launch_graph_op(const mpoa::GraphOpArgs& g, hipStream_t stream);

@@ -174,6 +174,9 @@ __device__ __forceinline__ void add_path(Arena& A, WinShared& S, const SlotCaps&
 // falls back to the serial walk. Returns false when that fallback is needed (nothing has been
 // modified then). stamp: N ints of scratch.
 template <int NT>
+__device__ __forceinline__ int block_excl_sum(int v, WinShared& S, int* total);
+
+template <int NT>
 __device__ __forceinline__ bool add_path_parallel(Arena& A, WinShared& S, const SlotCaps& c, bool keep_path,
                                   int seq_id, int m, int path0, int* cur, int* stamp) {
   bool dup = false;
@@ -2338,6 +2341,114 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
   if (tid == 0) state_store(S, *s.state);
 }
 
+// The call-level operations on a graph kept in device memory (GraphOpArgs): graph::init, one
+// poa() call on a rank window, graph::add_path of a given res, graph::topsort op 0 or 1 (with
+// the consensus the next read is seeded against), and the consensus, MSA or GFA output. The
+// same device functions as the batched pipeline, one step per launch.
+template <int NT, int CPL>
+__global__ __launch_bounds__(NT) void k_graph_op(GraphOpArgs g) {
+  __shared__ WinShared S;
+  __shared__ Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP> R;
+  using RingT = Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP>;
+  const KernelArgs& a = g.a;
+  const int tid = threadIdx.x;
+  Arena A = carve(a, 0);
+  uint8_t* lds = reinterpret_cast<uint8_t*>(&R);
+  if (g.op == GOP_INIT) {
+    window_begin<NT>(a, A, S, 0);  // graph::init: source 0 and sink 1 (graph.cpp:129-135)
+    if (tid == 0) {
+      state_store(S, *g.state);
+      g.out->status = S.status;
+    }
+    return;
+  }
+  if (tid == 0) state_load(S, *g.state);
+  __syncthreads();
+  if (g.op == GOP_POA) {
+    // poa() (align.cpp:1353-1737) on ranks [rank(beg_id), rank(end_id)]; the graph is read
+    // only (the DP plan, band state and matrices are scratch), so the state is not stored
+    if (tid == 0) {
+      S.status = MPOA_OK;
+      S.read = 0;
+      S.beg_i = A.rank[g.beg_id];
+      S.end_i = A.rank[g.end_id];
+      S.n_rows = S.end_i - S.beg_i + 1;
+      S.cells = 0;
+      S.edge_cells = 0;
+      for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
+    }
+    __syncthreads();
+    if (S.n_rows <= 0 || g.qlen > a.caps.qmax || S.n_rows > a.caps.nodes) {
+      if (tid == 0) {
+        g.out->status = S.n_rows <= 0 ? MPOA_E_EMPTY_WINDOW : MPOA_E_OVERFLOW;
+        g.out->rows = S.n_rows;
+        g.out->qlen = g.qlen;
+      }
+      return;
+    }
+    load_query<NT>(A, S, R, g.q, g.qlen);
+    if (S.q_lds) poa_call<NT, CPL, true, RingT>(a, A, S, R, g.adaptive != 0);
+    else poa_call<NT, CPL, false, RingT>(a, A, S, R, g.adaptive != 0);
+    if (tid == 0) {
+      GraphOpOut& o = *g.out;
+      o.status = S.status;
+      o.score = S.score;
+      o.rows = S.n_rows;
+      o.qlen = g.qlen;
+      o.width = S.call_width;
+      o.cells = S.call_cells;
+      o.edge_cells = S.call_ecells;
+    }
+    return;
+  }
+  if (g.op == GOP_ADD_PATH) {
+    // graph::add_path(m, seq_id, res, sink 1) (graph.cpp:170-235)
+    const bool keep_path = a.scores.result != 0;
+    const int m = g.n_res;
+    if (S.status == MPOA_OK) {
+      if (keep_path) {
+        for (int j = tid; j < m - 1; j += NT) A.path[g.path0 + j] = 1;
+        __syncthreads();
+      }
+      if ((a.forms & MPOA_FORM_ADD_LANE) ||
+          !add_path_parallel<NT>(A, S, a.caps, keep_path, g.seq_id, m, g.path0, A.t2, A.t1)) {
+        if (tid == 0) add_path(A, S, a.caps, keep_path, g.seq_id, m, g.path0);
+      }
+      __syncthreads();
+    }
+  } else if (g.op == GOP_TOPSORT) {
+    // graph::topsort(para, op) (graph.cpp:21-128); with seeding, build_consensus
+    if (S.status == MPOA_OK) {
+      if (g.sort_op == 0) {
+        topsort0<NT>(A, S, a.caps, a.scores.f > 0, lds, (int)sizeof(R), a.forms);
+        if (tid == 0) S.msa_cols = S.n_nodes;  // an MSA now has one column per rank
+        if (g.seeding) {
+          heaviest_ranked<NT>(A, S, lds, (int)sizeof(R), a.forms);
+          if (S.cons_len > g.cons_cap) {
+            if (tid == 0) S.status = MPOA_E_OVERFLOW;
+          } else {
+            for (int p = tid; p < S.cons_len; p += NT) {
+              g.cons_out[3 * p] = A.cons_id[p];
+              g.cons_out[3 * p + 1] = A.cons_rank[p];
+              g.cons_out[3 * p + 2] = A.cons_ch[p];
+            }
+          }
+        }
+      } else if (tid == 0) {
+        S.msa_cols = topsort1(A, S);
+      }
+      __syncthreads();
+    }
+  } else if (g.op == GOP_OUTPUT) {
+    // graph::output_consensus / output_rc_msa / output_gfa (graph.cpp:237-433)
+    window_outputs<NT>(a, A, S, 0, lds, (int)sizeof(R));
+  }
+  if (tid == 0) {
+    state_store(S, *g.state);
+    g.out->status = S.status;
+  }
+}
+
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
   __shared__ WinShared S;
@@ -2518,5 +2629,11 @@ int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t s
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream) {
   using namespace mpoa;
   hipLaunchKernelGGL((k_commit<256>), dim3(1), dim3(256), 0, stream, s);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mpoa_launch_graph_op(const mpoa::GraphOpArgs& g, hipStream_t stream) {
+  using namespace mpoa;
+  hipLaunchKernelGGL((k_graph_op<512, 2>), dim3(1), dim3(512), 0, stream, g);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

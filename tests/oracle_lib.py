@@ -162,3 +162,121 @@ def dp_matrix(p, seq: np.ndarray, off: np.ndarray) -> dict:
         raise RuntimeError(f"oracle error {rc}")
     m["n_rows"], m["score"] = R, score.value
     return m
+
+
+class Dag:
+    """The oracle's call-level mirror (orc_dag_*): minipoa's poa(), alignment(),
+    graph::add_path, graph::topsort and outputs on one graph. res arrays are int32 (n, 3):
+    from, base code, aligned_id."""
+
+    def __init__(self, p, seq: np.ndarray | None = None, off: np.ndarray | None = None):
+        L = lib()
+        L.orc_dag_new.restype = ctypes.c_void_p
+        L.orc_dag_new.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                  ctypes.c_void_p]
+        L.orc_dag_free.argtypes = [ctypes.c_void_p]
+        L.orc_dag_free.restype = None
+        self.op = oparams(p)
+        self._seq = None if seq is None else np.ascontiguousarray(seq, np.uint8)
+        self._off = None if off is None else np.ascontiguousarray(off, np.int64)
+        n = 0 if off is None else len(off) - 1
+        self.h = L.orc_dag_new(ctypes.byref(self.op), n, _p(self._seq), _p(self._off))
+        if not self.h:
+            raise RuntimeError("orc_dag_new failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_dag_free(self.h)
+            self.h = None
+
+    def _res(self, fn, *args) -> np.ndarray:
+        n = ctypes.c_int64()
+        rc = fn(ctypes.c_void_p(self.h), *args, None, ctypes.c_int64(0), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        out = np.zeros((max(n.value, 1), 3), np.int32)
+        rc = fn(ctypes.c_void_p(self.h), *args, _p(out), ctypes.c_int64(len(out)),
+                ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        return out[:n.value]
+
+    def poa(self, beg_id: int, end_id: int, q: bytes, ab_band: bool):
+        score = ctypes.c_int32()
+        n = ctypes.c_int64()
+        buf = ctypes.create_string_buffer(q, len(q) + 1)
+        out = np.zeros((len(q) + 2, 3), np.int32)
+        rc = lib().orc_dag_poa(ctypes.c_void_p(self.h), beg_id, end_id, buf, len(q),
+                               int(ab_band), _p(out), ctypes.c_int64(len(out)), ctypes.byref(n),
+                               ctypes.byref(score))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        return out[:n.value], score.value
+
+    def alignment(self, rid: int, q: bytes) -> np.ndarray:
+        buf = ctypes.create_string_buffer(q, len(q) + 1)
+        return self._res(lib().orc_dag_alignment, rid, buf, len(q))
+
+    def add_path(self, seq_id: int, res: np.ndarray) -> None:
+        res = np.ascontiguousarray(res, np.int32)
+        rc = lib().orc_dag_add_path(ctypes.c_void_p(self.h), seq_id, _p(res),
+                                    ctypes.c_int64(len(res)))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+
+    def topsort(self, op: int) -> None:
+        if lib().orc_dag_topsort(ctypes.c_void_p(self.h), op) != 0:
+            raise RuntimeError("oracle topsort failed")
+
+    def output(self, result: int, names=None, comments=None) -> str:
+        nb, no = _names(names)
+        cb, co = _names(comments)
+        n = ctypes.c_int64()
+        L = lib()
+        rc = L.orc_dag_output(ctypes.c_void_p(self.h), result, _p(nb), _p(no), _p(cb), _p(co),
+                              None, ctypes.c_int64(0), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        buf = np.zeros(max(n.value, 1), np.uint8)
+        rc = L.orc_dag_output(ctypes.c_void_p(self.h), result, _p(nb), _p(no), _p(cb), _p(co),
+                              _p(buf), ctypes.c_int64(len(buf)), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        return buf[:n.value].tobytes().decode()
+
+    def export(self) -> dict:
+        L = lib()
+        nn, ne = ctypes.c_int32(), ctypes.c_int32()
+        rc = L.orc_dag_export(ctypes.c_void_p(self.h), ctypes.byref(nn), ctypes.byref(ne),
+                              *([None] * 12))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        N, E = nn.value, ne.value
+        g = dict(base=np.zeros(N, np.uint8), in_off=np.zeros(N + 1, np.int32),
+                 in_src=np.zeros(E, np.int32), in_w=np.zeros(E, np.int32),
+                 out_off=np.zeros(N + 1, np.int32), out_dst=np.zeros(E, np.int32),
+                 out_w=np.zeros(E, np.int32), rank_to_id=np.zeros(N, np.int32),
+                 hlen=np.zeros(N, np.int32), tlen=np.zeros(N, np.int32),
+                 par_id=np.zeros(N, np.int32), aligned=np.zeros(5 * N, np.int32))
+        order = ["base", "in_off", "in_src", "in_w", "out_off", "out_dst", "out_w",
+                 "rank_to_id", "hlen", "tlen", "par_id", "aligned"]
+        rc = L.orc_dag_export(ctypes.c_void_p(self.h), ctypes.byref(nn), ctypes.byref(ne),
+                              *[_p(g[k]) for k in order])
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        return g
+
+
+def run_loop(dag, n_reads: int, seq: np.ndarray, off: np.ndarray, result: int,
+             names=None, comments=None, seeder=None):
+    """minipoa's main loop (src/main.cpp:110-126) driven through a call-level object (the
+    oracle's Dag or the product's api.Graph): per read alignment -> add_path -> topsort
+    (op 1 after the last read), then the output. Returns (text, [res of every read])."""
+    all_res = []
+    for i in range(n_reads):
+        q = bytes(seq[off[i]:off[i + 1]])
+        res = dag.alignment(i, q) if seeder is None else dag.alignment(i, q, seeder)
+        all_res.append(res)
+        dag.add_path(i, res)
+        dag.topsort(1 if i + 1 == n_reads else 0)
+    return dag.output(result, names, comments), all_res

@@ -24,8 +24,27 @@ def declared(header: str) -> list[str]:
     return sorted(set(DECL.findall(text)))
 
 
+# header-only C++ facades and the C header whose library they call
+FACADES = {"minipoa_amd.hpp": "mpoa.h"}
+
+
 def test_every_header_has_a_library():
-    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == sorted(LIBS)
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == sorted(list(LIBS) + list(FACADES))
+
+
+@pytest.mark.parametrize("facade", sorted(FACADES))
+def test_facade_compiles(facade, tmp_path):
+    """The facade compiles on its own and calls only functions its C header declares."""
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text(f'#include "{facade}"\nint main() {{ return 0; }}\n')
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        f"-I{os.path.join(ROOT, 'include')}", str(src)],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    with open(os.path.join(ROOT, "include", facade)) as f:
+        called = set(re.findall(r"\b(mpoa_\w+)\s*\(", f.read()))
+    assert called and called <= set(declared(FACADES[facade]))
 
 
 @pytest.mark.parametrize("header", sorted(LIBS))
