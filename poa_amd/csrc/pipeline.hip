@@ -64,6 +64,7 @@ struct WinShared {
   int job;
   int scan[16];
   int row_bs, row_be;
+  int row_wmax;   // widest row of the current static-band plan (8*block_num)
   bool q_lds;
   int q_cap;
   int64_t ticks[PH_N];
@@ -644,7 +645,7 @@ __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps&
     const int k0 = A.pred_off[r];
     const int ind10 = A.ind[u] / 10;
     int k = k0, wmax = -1, max_pre = -1, pr3[3] = {0, 0, 0}, w3[3] = {0, 0, 0}, pk = 0;
-    int omax = -1, max_suc = -1;
+    int omax = -1, max_suc = -1, last_suc = -1;
     // the in-list and the out-list are walked in one loop, so that their loads overlap
     int ei = A.in_head[u], eo = A.out_head[u];
     while (ei != -1 || eo != -1) {
@@ -668,6 +669,7 @@ __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps&
         const int sr = A.rank[A.oe_dst[eo]], ow = A.oe_w[eo];
         eo = A.oe_next[eo];
         if (ow > omax) { omax = ow; max_suc = sr; }
+        last_suc = max(last_suc, sr);
       }
     }
     int* rec = A.rowrec + 12LL * r;
@@ -677,6 +679,7 @@ __device__ __forceinline__ void topsort0(Arena& A, WinShared& S, const SlotCaps&
     rec[8] = w3[0];
     rec[9] = w3[1];
     rec[10] = w3[2];
+    rec[11] = last_suc;
     if (lds_len) {
       lpar[r] = (int16_t)max_pre;
       lchd[r] = (int16_t)max_suc;
@@ -753,9 +756,9 @@ __device__ __forceinline__ int topsort1(Arena& A, WinShared& S) {
 //   [7] bits 0-8: their bases (3 bits each); bits 9-11: their in-weight >= ind/10 (the
 //       backtrack's match rule, src/align.cpp:1609)
 //   [8..10] in-weights of the first three predecessors
-//   [11] unused (keeps the record three 16-byte vectors)
+//   [11] the largest rank among the successors (-1: none): the last row that reads this one
 struct RowRec {
-  int row_off, bsbe, k0, np, p0, p1, p2, pk, w0, w1, w2, spare;
+  int row_off, bsbe, k0, np, p0, p1, p2, pk, w0, w1, w2, last;
 };
 __device__ __forceinline__ RowRec rowrec_of(const int4& a, const int4& b, const int4& c) {
   return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
@@ -928,28 +931,28 @@ __device__ __forceinline__ void load_q(P qs, int cb, int cap, int* qc) {
 
 
 // One predecessor's M/D-source contribution to this thread's columns (the band conditions of
-// src/align.cpp:1467 and :1484 as contiguous column ranges), from the LDS ring or from HBM.
-// The ring's row index and band of LDS slot k live in lane k of two registers (meta_row,
-// meta_bsbe; every wave keeps the same copy), so looking a predecessor up costs no LDS access.
-__device__ __forceinline__ bool ring_has(int meta_row, int i, int p, int ring) {
-  const int slot = __builtin_amdgcn_readfirstlane(p % ring);
-  return i - p < ring && __builtin_amdgcn_readlane(meta_row, slot) == p;
+// src/align.cpp:1467 and :1484 as contiguous column ranges), from the LDS row cache or from
+// HBM. The cache directory lives in registers, lane k describing slot k (every wave keeps the
+// same copy): meta_row, the local row the slot holds (-1: none); meta_bsbe, its band; a
+// lookup is one compare and a ballot, with no LDS access. Slot k holds its row's band at
+// R.H / R.D + k * sw.
+__device__ __forceinline__ int cache_slot(int meta_row, int p) {
+  const uint64_t hit = __ballot(meta_row == p);
+  return hit ? (int)__builtin_ctzll(hit) : -1;
 }
 
 template <int CPL, class RingT>
-__device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpScores& sc, int i,
-                                             int p, int pbase, int cb, int c0, int c1,
-                                             const int* qc, int* h, int* d, int meta_row,
-                                             int meta_bsbe) {
-  constexpr int RING = RingT::kRing, WR = RingT::kWR;
+__device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpScores& sc, int p,
+                                             int pbase, int cb, int c0, int c1, const int* qc,
+                                             int* h, int* d, int meta_row, int meta_bsbe, int sw) {
   if (p == 0) pbase = CODE_N;
-  const int slot = __builtin_amdgcn_readfirstlane(p % RING);
-  if (ring_has(meta_row, i, p, RING)) {
+  const int slot = cache_slot(meta_row, p);
+  if (slot >= 0) {
     const int bsbe = __builtin_amdgcn_readlane(meta_bsbe, slot);
     const int bs = bsbe & 0xffff, be = bsbe >> 16;
     const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
     const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
-    accumulate<CPL>(R.H + slot * WR, R.D + slot * WR, WR, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
+    accumulate<CPL>(R.H + slot * sw, R.D + slot * sw, sw, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
                     pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
   } else {
     const int bs = A.Bs[p], be = A.Be[p], off = A.row_off[p];
@@ -966,26 +969,25 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
 //   I[j] = j*e + max(NEG_INF, max_{k<j} (G[k] + o - (k+1)*e)),  H = max(G, I),
 // G = max(M-source, D-source). This equals the serial recurrence of align.cpp:1507-1514,
 // I[j] = max(I[j-1]+e, H[j-1]+o), because o <= e (gap_open <= 0) and int32 never wraps here.
-// Writes H, D, I (and the NEG_INF sentinel block) to HBM and H, D to the LDS ring. The common
-// path reads LDS only: on gfx9 a wait for any HBM load also waits for every store issued
-// before it, so HBM reads here (predecessors outside the ring, predecessors after the third,
-// a query too long for LDS) sit in branches of their own.
+// Writes H, D, I (and the NEG_INF sentinel block) to HBM and, when the row has a cache slot
+// (slot_i >= 0), H and D to LDS. The common path reads LDS only: on gfx9 a wait for any HBM
+// load also waits for every store issued before it, so HBM reads here (predecessors outside
+// the cache, predecessors after the third, a query too long for LDS) sit in branches of their
+// own.
 // Returns the first argmax column (local) of the row's H, for the adaptive band; *np_in gets
 // the number of in-window predecessors.
 template <int NT, int CPL, bool QLDS, class RingT>
 __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
                       const RowRec& rr, int bs, int be, bool want_argmax, int* np_in,
-                      int& meta_row, int& meta_bsbe, int& scan_ph) {
-  constexpr int RING = RingT::kRing, WR = RingT::kWR;
+                      int meta_row, int meta_bsbe, int sw, int slot_i, int& scan_ph) {
   const int tid = threadIdx.x;
   const int c0 = bs * Q, W = (be - bs - 1) * Q, c1 = c0 + W;
   int* Hi = A.H + rr.row_off;  // column c at Hi[c - c0]
   int* Di = A.D + rr.row_off;
   int* Ii = A.I + rr.row_off;
-  const int slot_i = i % RING;
-  const bool cache_i = W <= WR;
-  int* RHi = R.H + slot_i * WR;  // column c at RHi[c - c0]
-  int* RDi = R.D + slot_i * WR;
+  const bool cache_i = slot_i >= 0;
+  int* RHi = R.H + (cache_i ? slot_i : 0) * sw;  // column c at RHi[c - c0]
+  int* RDi = R.D + (cache_i ? slot_i : 0) * sw;
   const int o = sc.o1, e = sc.e1;
   const int beg = S.beg_i;
   const int n3 = min(rr.np, 3);
@@ -1006,8 +1008,8 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
       const int p = prank - beg;
       if (p < 0) continue;
       npi++;
-      pred_contrib<CPL>(A, R, sc, i, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d,
-                        meta_row, meta_bsbe);
+      pred_contrib<CPL>(A, R, sc, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d, meta_row,
+                        meta_bsbe, sw);
     }
     if (rr.np > 3) {
       for (int k = 3; k < rr.np; k++) {
@@ -1016,7 +1018,7 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
         drain_vmem();
         if (p < 0) continue;
         npi++;
-        pred_contrib<CPL>(A, R, sc, i, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe);
+        pred_contrib<CPL>(A, R, sc, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe, sw);
       }
       drain_vmem();
     }
@@ -1056,11 +1058,6 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
   *np_in = npi;
   // NEG_INF sentinel block after the band (align.cpp:1505-1506,1515).
   if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = NEG_INF;
-  {
-    const int lane = tid & 63;
-    meta_row = lane == slot_i ? (cache_i ? i : -1) : meta_row;
-    meta_bsbe = lane == slot_i ? (bs | (be << 16)) : meta_bsbe;
-  }
   if (!want_argmax) return 0;
   // First argmax over the row (value desc, column asc).
 #pragma unroll
@@ -1110,6 +1107,7 @@ __device__ __forceinline__ bool plan_static(const Arena& A, WinShared& S, const 
       A.Bs[r] = bs;
       A.Be[r] = be;
       len = (be - bs) * Q;
+      atomicMax(&S.row_wmax, len - Q);
     }
     // prefix sum via shuffles + shared
     int v = len;
@@ -1546,6 +1544,8 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
   const int64_t t0 = now();
   const int full_len = (m / Q + 2) * Q;
   if (!adaptive) {
+    if (tid == 0) S.row_wmax = 0;
+    __syncthreads();
     if (!plan_static<NT>(A, S, sc, a.caps, w)) {
       if (tid == 0) S.status = MPOA_E_OVERFLOW;
       __syncthreads();
@@ -1568,18 +1568,34 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     }
     __syncthreads();
   }
-  // Ring directory: lane k holds the row index (-1: none) and band of ring slot k.
-  int meta_row = -1, meta_bsbe = 0;
+  // LDS row cache: slots of sw columns (H and D each), as many as fit the LDS arrays (at most
+  // 64, one per lane of the directory), sw the widest row of the call (static band) or a
+  // typical adaptive band. A row takes a slot when a later row reads it and it fits; it keeps
+  // the slot until its last reader (the largest successor rank, in its row record) is done.
+  // With no slot free, it takes the slot whose row is read last, if that is read after its
+  // own last reader, and never a slot the row itself reads. Rows without a slot are read from
+  // HBM.
+  const int lane = tid & 63;
+  int sw;
+  if (!adaptive) {
+    sw = S.row_wmax;
+  } else {
+    sw = min(full_len - Q, (2 * w + 128 + Q - 1) / Q * Q);
+  }
+  sw = max(sw, Q);
+  const int ns = min(64, (RING * WR) / sw);
+  int meta_row = -1, meta_bsbe = 0, meta_last = -1;
   int scan_ph = 0;  // half of S.scan the next row scan writes
   // Row 0 (align.cpp:1415-1425): M, D = NEG_INF over blocks [Bs0, Be0) from the row start;
   // M[0][0] = 0. I of row 0 is never read; it is set to NEG_INF. Bs0 is always 0.
   {
     const int be0 = A.Be[0];
     const int len0 = be0 * Q, w0 = (be0 - 1) * Q;
+    const int last0 = A.rowrec[12LL * beg + 11];
     int* H0 = A.H + A.row_off[0];
     int* D0 = A.D + A.row_off[0];
     int* I0 = A.I + A.row_off[0];
-    const bool cache0 = w0 <= WR;
+    const bool cache0 = ns > 0 && w0 <= sw && last0 > beg;
     for (int j = tid; j < len0; j += NT) {
       H0[j] = j == 0 ? 0 : NEG_INF;
       D0[j] = NEG_INF;
@@ -1589,9 +1605,11 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
         R.D[j] = NEG_INF;
       }
     }
-    const int lane = tid & 63;
-    meta_row = lane == 0 && cache0 ? 0 : -1;
-    meta_bsbe = be0 << 16;
+    if (cache0 && lane == 0) {
+      meta_row = 0;
+      meta_bsbe = be0 << 16;
+      meta_last = last0 - beg;
+    }
   }
   __syncthreads();
   const int64_t t1 = now();
@@ -1634,27 +1652,53 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
       __syncthreads();
       return;
     }
-    // Predecessors outside the ring are read from HBM: with several waves, their stores must
-    // have completed first (one wave sees its own stores in order).
-    if (NT > 64) {
-      bool far = false;
+    // Slots this row reads, and whether a predecessor comes from HBM: with several waves, its
+    // stores must have completed first (one wave sees its own stores in order).
+    uint64_t pmask = 0;
+    bool far = false;
+    {
       const int n3 = min(rr.np, 3);
       for (int k = 0; k < n3; k++) {
         const int p = (k == 0 ? rr.p0 : k == 1 ? rr.p1 : rr.p2) - beg;
-        if (p >= 0) far |= !ring_has(meta_row, i, p, RING);
+        if (p < 0) continue;
+        const int sl = cache_slot(meta_row, p);
+        if (sl >= 0) pmask |= 1ull << sl;
+        else far = true;
       }
-      if (rr.np > 3) {
+      if (NT > 64 && rr.np > 3) {
         for (int k = 3; k < rr.np; k++) {
           const int p = A.pred_rank[rr.k0 + k] - beg;
           drain_vmem();
-          if (p >= 0) far |= !ring_has(meta_row, i, p, RING);
+          if (p >= 0) far |= cache_slot(meta_row, p) < 0;
         }
       }
-      if (far) __syncthreads();
+      if (NT > 64 && far) __syncthreads();
+    }
+    // A slot for this row: a free one, else (predecessors after the third unknown here: no
+    // eviction then) the one whose row is read last.
+    const int last_i = rr.last < 0 ? -1 : rr.last - beg;
+    int slot_i = -1;
+    if (W <= sw && last_i > i) {
+      const uint64_t freem = __ballot(lane < ns && meta_last < i);
+      if (freem) {
+        slot_i = (int)__builtin_ctzll(freem);
+      } else if (rr.np <= 3) {
+        const int v = (lane < ns && !((pmask >> lane) & 1)) ? meta_last : -1;
+        int mx = v;
+#pragma unroll
+        for (int sh = 32; sh > 0; sh >>= 1) mx = max(mx, __shfl_xor(mx, sh, 64));
+        mx = __builtin_amdgcn_readfirstlane(mx);
+        if (mx > last_i) slot_i = (int)__builtin_ctzll(__ballot(v == mx));
+      }
     }
     int np_in = 0;
-    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in,
-                                         meta_row, meta_bsbe, scan_ph);
+    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in, meta_row,
+                                         meta_bsbe, sw, slot_i, scan_ph);
+    if (slot_i >= 0 && lane == slot_i) {
+      meta_row = i;
+      meta_bsbe = bs | (be << 16);
+      meta_last = last_i;
+    }
     lds_barrier<NT>();
     cells += W;
     ecells += (int64_t)np_in * W;
