@@ -26,7 +26,10 @@ typedef struct mpoa_params {
   int32_t seeding;                            /* -S */
   int32_t k, mm_w;                            /* -k -w (defaults 19 10) */
   int32_t poa_w;                              /* -W; 0 = flag not given */
-  int32_t result;                             /* -r: 0 consensus, 1 row-column MSA, 2 GFA */
+  int32_t result;                             /* -r: 0 consensus, 1 row-column MSA, 2 GFA;
+                                                 3 (no minipoa counterpart): the -r 1 MSA,
+                                                 then a consensus row, ">consensus sequence",
+                                                 each consensus base in its node's column */
 } mpoa_params;
 
 /* Error codes. minipoa terminates the process in these cases; the library returns. */
@@ -148,7 +151,8 @@ int mpoa_graph_add_path(mpoa_graph* g, int32_t seq_id, const mpoa_res* res, int6
  * aligned groups (MSA columns). */
 int mpoa_graph_topsort(mpoa_graph* g, int32_t op);
 
-/* graph::output_consensus (result 0), output_rc_msa (1) or output_gfa (2)
+/* graph::output_consensus (result 0), output_rc_msa (1) or output_gfa (2), or the MSA with a
+ * consensus row (3)
  * (src/graph.cpp:237-433, called at src/main.cpp:124-126) over the reads added so far, as the
  * text minipoa prints; names / comments as for mpoa_align_reads. The MSA of a graph not sorted
  * since its last add_path is taken after an op 1 sort, as output_rc_msa does. *out_len

@@ -1171,6 +1171,27 @@ static void output_msa(Graph& G, const Params& P, int n_reads, const std::vector
   }
 }
 
+// The row-column MSA (-r 1) followed by a consensus row: the consensus of build_consensus
+// (graph.cpp:294-357), each base at the column of its node's aligned group (the op 1 rank that
+// places the node's base in the read rows, graph.cpp:240-245), '-' elsewhere. minipoa has no
+// such output; its rows are the -r 1 rows and its ungapped row the -r 0 consensus.
+static void output_msa_cons(Graph& G, const Params& P, int n_reads,
+                            const std::vector<std::string>& name,
+                            const std::vector<std::string>& comment, Text& o) {
+  output_msa(G, P, n_reads, name, comment, o);
+  const size_t L = G.rank.size() >= 2 ? G.rank.size() - 2 : 0;
+  std::string row(L, '-');
+  G.build_consensus();
+  for (size_t p = 0; p < G.cons_pos_to_id.size(); p++) {
+    const int id = G.cons_pos_to_id[p];
+    const int rk = G.node[G.node[id].par_id].rank;
+    if (rk >= 1 && (size_t)rk <= L) row[rk - 1] = char_of(G.node[id].base);
+  }
+  o.put(">consensus sequence\n");
+  o.put(row);
+  o.put('\n');
+}
+
 // output_gfa (graph.cpp:358-433)
 static void output_gfa(const Graph& G, int n_reads, const std::vector<int>& read_len,
                        const std::vector<std::string>& name, Text& o) {
@@ -1261,6 +1282,8 @@ static int run_window(const Params& P, int n_reads, const char* seq, const int64
     o->put('\n');
   } else if (P.result == 1) {
     output_msa(G, P, n_reads, name, comment, *o);
+  } else if (P.result == 3) {
+    output_msa_cons(G, P, n_reads, name, comment, *o);
   } else if (P.result == 2) {
     std::vector<int> len(n_reads);
     for (int i = 0; i < n_reads; i++) len[i] = (int)(off[i + 1] - off[i]);
@@ -1574,7 +1597,7 @@ extern "C" int orc_dag_output(orc_dag* g, int32_t result, const char* name,
                                 const int64_t* name_off, const char* comment,
                                 const int64_t* comment_off, char* out, int64_t out_cap,
                                 int64_t* out_len) {
-  if (!g || result < 0 || result > 2) return ORC_E_ARG;
+  if (!g || result < 0 || result > 3) return ORC_E_ARG;
   const int n = (int)g->read_len.size();
   const std::vector<std::string> nm = orc::split(name, name_off, n);
   const std::vector<std::string> cm = orc::split(comment, comment_off, n);
@@ -1586,6 +1609,8 @@ extern "C" int orc_dag_output(orc_dag* g, int32_t result, const char* name,
     o.put('\n');
   } else if (result == 1) {
     orc::output_msa(g->G, g->P, n, nm, cm, o);
+  } else if (result == 3) {
+    orc::output_msa_cons(g->G, g->P, n, nm, cm, o);
   } else {
     orc::output_gfa(g->G, n, g->read_len, nm, o);
   }

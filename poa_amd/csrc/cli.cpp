@@ -90,7 +90,7 @@ void usage() {
                "  -M INT match (2)      -X INT mismatch (-4)  -O INT gap open (-4)\n"
                "  -E INT gap extend (-2) -b INT band b (100)  -f INT band f (40)\n"
                "  -B adaptive band      -S seeding           -k INT k-mer (19)\n"
-               "  -w INT window (10)    -W INT sub-window     -r INT 0 consensus, 1 MSA, 2 GFA\n"
+               "  -w INT window (10)    -W INT sub-window     -r INT 0 consensus, 1 MSA, 2 GFA, 3 MSA + consensus row\n"
                "  -V INT verbosity      -t INT threads (ignored)\n");
 }
 

@@ -146,6 +146,7 @@ int regrow(mpoa_graph* g, SlotCaps nc) {
 // are refreshed.
 int run_op(mpoa_graph* g, GraphOpArgs& op, GraphOpOut* out) {
   op.a = args_of(g);
+  if (op.op == mpoa::GOP_OUTPUT) op.a.scores.result = op.result;  // the output asked for
   op.state = g->d_state.p;
   op.out = g->d_out.p;
   op.cons_out = g->d_cons.p;
@@ -266,7 +267,7 @@ int64_t text_cap(int result, const std::vector<int>& len) {
     qmax = std::max<int64_t>(qmax, l);
   }
   const int64_t n = (int64_t)len.size();
-  if (result == 1) return n * (2 + qtot) + 64;
+  if (result == 1 || result == 3) return (n + 1) * (2 + qtot) + 64;
   if (result == 2) {
     int d = 1;
     for (int64_t v = 2 + qtot; v >= 10; v /= 10) d++;
@@ -282,8 +283,8 @@ extern "C" {
 int mpoa_graph_create(mpoa_ctx* ctx, const mpoa_params* params, mpoa_graph** out) {
   if (!ctx || !params || !out) return MPOA_E_ARG;
   *out = nullptr;
-  if (params->result < 0 || params->result > 2) {
-    g_err = "result must be 0 (consensus), 1 (MSA) or 2 (GFA)";
+  if (params->result < 0 || params->result > 3) {
+    g_err = "result must be 0 (consensus), 1 (MSA), 2 (GFA) or 3 (MSA with a consensus row)";
     return MPOA_E_ARG;
   }
   HIPCHK(hipSetDevice(ctx->device));
@@ -468,15 +469,16 @@ int mpoa_graph_topsort(mpoa_graph* g, int32_t op) {
 int mpoa_graph_output(mpoa_graph* g, int32_t result, const char* name, const int64_t* name_off,
                       const char* comment, const int64_t* comment_off, char* out, int64_t out_cap,
                       int64_t* out_len) {
-  if (!g || result < 0 || result > 2) return MPOA_E_ARG;
+  if (!g || result < 0 || result > 3) return MPOA_E_ARG;
   if (result != 0 && g->params.result == 0) {
-    g_err = "MSA and GFA output need a graph created with result 1 or 2 (read paths kept)";
+    g_err = "MSA and GFA output need a graph created with result 1, 2 or 3 (read paths kept)";
     return MPOA_E_ARG;
   }
   HIPCHK(hipSetDevice(g->ctx->device));
   int rc;
   // output_rc_msa sorts an unsorted graph by aligned groups first (graph.cpp:238)
-  if (result == 1 && g->sorted < 0 && (rc = do_topsort(g, 1)) != MPOA_OK) return rc;
+  if ((result == 1 || result == 3) && g->sorted < 0 && (rc = do_topsort(g, 1)) != MPOA_OK)
+    return rc;
   const int n = (int)g->read_len.size();
   const int64_t cap = std::min<int64_t>(text_cap(result, g->read_len), INT32_MAX / 2);
   if (!g->d_text.p || g->d_text.n < (size_t)cap)
@@ -499,11 +501,12 @@ int mpoa_graph_output(mpoa_graph* g, int32_t result, const char* name, const int
   // else the nodes (rank.size() - 2 columns, graph.cpp:239)
   GraphOpArgs op{};
   op.op = mpoa::GOP_OUTPUT;
+  op.result = result;
   GraphOpOut o{};
   if ((rc = run_op(g, op, &o)) != MPOA_OK) return rc;
   if ((rc = check_status(o.status)) != MPOA_OK) return rc;
   // the consensus output re-sorts the graph (op 0 without hlen/tlen): a later poa() needs a sort
-  if (result == 0 && g->sorted != 0) g->sorted = -1;
+  if ((result == 0 || result == 3) && g->sorted != 0) g->sorted = -1;
   WinResult r{};
   HIPCHK(hipMemcpy(&r, g->d_res.p, sizeof r, hipMemcpyDeviceToHost));
   std::string text;

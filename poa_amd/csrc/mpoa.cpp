@@ -834,11 +834,12 @@ int window_text(int result, const WinResult& r, const uint8_t* d_out, const int6
       }
     }
   } else {
-    // output_rc_msa (src/graph.cpp:237-251)
+    // output_rc_msa (src/graph.cpp:237-251); result 3 adds the consensus row
     const int L = r.out_len;
-    std::vector<char> rows((size_t)std::max<int64_t>((int64_t)n_reads * L, 1));
-    if ((int64_t)n_reads * L > 0 &&
-        hipMemcpy(rows.data(), d_out, (size_t)n_reads * L, hipMemcpyDeviceToHost) !=
+    const int n_rows = n_reads + (result == 3 ? 1 : 0);
+    std::vector<char> rows((size_t)std::max<int64_t>((int64_t)n_rows * L, 1));
+    if ((int64_t)n_rows * L > 0 &&
+        hipMemcpy(rows.data(), d_out, (size_t)n_rows * L, hipMemcpyDeviceToHost) !=
             hipSuccess) {
       return MPOA_E_DEVICE;
     }
@@ -849,6 +850,11 @@ int window_text(int result, const WinResult& r, const uint8_t* d_out, const int6
       if (nm.empty()) text += ">" + std::to_string(i + 1) + "\n";
       else text += ">" + nm + " " + cm + "\n";
       text.append(rows.data() + (int64_t)i * L, (size_t)L);
+      text.push_back('\n');
+    }
+    if (result == 3) {
+      text += ">consensus sequence\n";
+      text.append(rows.data() + (int64_t)n_reads * L, (size_t)L);
       text.push_back('\n');
     }
   }
@@ -935,8 +941,8 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
     g_err = "seeding (-S) windows are not supported by the batch pipeline";
     return MPOA_E_ARG;
   }
-  if (params->result < 0 || params->result > 2) {
-    g_err = "result must be 0 (consensus), 1 (MSA) or 2 (GFA)";
+  if (params->result < 0 || params->result > 3) {
+    g_err = "result must be 0 (consensus), 1 (MSA), 2 (GFA) or 3 (MSA with a consensus row)";
     return MPOA_E_ARG;
   }
   HIPCHK(hipSetDevice(ctx->device));
@@ -966,8 +972,8 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
     d.n_reads = wi.n_reads;
     d.out0 = out0;
     int64_t cap;
-    if (params->result == 1) {
-      cap = (int64_t)wi.n_reads * (2 + wi.qtot);
+    if (params->result == 1 || params->result == 3) {
+      cap = (int64_t)(wi.n_reads + 1) * (2 + wi.qtot);
     } else if (params->result == 2) {
       // output_gfa bound: nodes <= 2 + sum q, in-edges <= sum q + reads, d-digit ids
       int d = 1;
@@ -1181,6 +1187,11 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
       for (int i = 0; i < b->info[w].n_reads; i++) {
         text += ">" + std::to_string(i + 1) + "\n";
         text.append((const char*)src + (int64_t)i * L, (size_t)L);
+        text.push_back('\n');
+      }
+      if (b->params.result == 3) {
+        text += kHead;
+        text.append((const char*)src + (int64_t)b->info[w].n_reads * L, (size_t)L);
         text.push_back('\n');
       }
     }

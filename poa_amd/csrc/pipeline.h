@@ -216,6 +216,7 @@ struct GraphOpArgs {
   int sort_op, seeding;          // GOP_TOPSORT: 0 or 1; with seeding, the consensus too
   int* cons_out;                 // (node id, rank, base) per consensus position
   int cons_cap;
+  int result;                    // GOP_OUTPUT: 0 consensus, 1 MSA, 2 GFA, 3 MSA + consensus
 };
 
 __host__ __device__ inline Arena carve(const KernelArgs& a, int slot) {

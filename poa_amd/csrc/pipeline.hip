@@ -2104,11 +2104,14 @@ __device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, Wi
         for (int at = tid; at < len; at += NT) o[at] = char_of(A.cons_ch[at]);
       }
       __syncthreads();
-    } else if (a.scores.result == 1) {
-      // Row-column MSA (graph.cpp:237-251): rows = reads, columns = group ranks 1..R-2.
+    } else if (a.scores.result == 1 || a.scores.result == 3) {
+      // Row-column MSA (graph.cpp:237-251): rows = reads, columns = group ranks 1..R-2. With
+      // result 3 one more row follows: the consensus (graph.cpp:294-357) with each base in
+      // its node's column.
       const int R = S.msa_cols;
       const int L = R >= 2 ? R - 2 : 0;
-      const int64_t need = (int64_t)wd.n_reads * L;
+      const bool cons_row = a.scores.result == 3;
+      const int64_t need = (int64_t)(wd.n_reads + (cons_row ? 1 : 0)) * L;
       if (need > wd.out_cap) {
         if (tid == 0) S.status = MPOA_E_OVERFLOW;
       } else {
@@ -2122,11 +2125,29 @@ __device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, Wi
             const int v = A.path[p0 + pos];
             if (v >= 2) {
               const int rk = A.rank[A.par[v]];
-              o[(int64_t)r * L + rk - 1] = char_of(A.base[v]);
+              if (rk >= 1 && rk <= L) o[(int64_t)r * L + rk - 1] = char_of(A.base[v]);
             }
           }
           p0 += qlen;
         }
+        if (cons_row) {
+          // the columns are the aligned-group ranks of the op 1 sort; the consensus needs the
+          // op 0 order of the final graph, which overwrites them: keep them in A.Mp (DP
+          // scratch) first
+          for (int v = tid; v < S.n_nodes; v += NT) A.Mp[v] = A.rank[v];
+          __syncthreads();
+          topsort0<NT>(A, S, a.caps, false, lds, lds_bytes, a.forms);
+          heaviest_ranked<NT>(A, S, lds, lds_bytes, a.forms);
+          uint8_t* row = o + (int64_t)wd.n_reads * L;
+          for (int x = tid; x < L; x += NT) row[x] = '-';
+          __syncthreads();
+          for (int at = tid; at < S.cons_len; at += NT) {
+            const int id = A.cons_id[at];
+            const int rk = A.Mp[A.par[id]];
+            if (rk >= 1 && rk <= L) row[rk - 1] = char_of(A.base[id]);
+          }
+        }
+        __syncthreads();
         if (tid == 0) S.cons_len = L;
       }
       __syncthreads();

@@ -4,7 +4,9 @@
 20 kb x 50 read seeded run), each generated by poa_amd.synth from its seed and run through the
 upstream minipoa binary (oracle/_ref/minipoa, built by `make -C oracle ref`). The fixture
 stores the generator call, a SHA-256 of the generated FASTA (so a changed generator is caught)
-and the upstream outputs: stdout and every DP call's score (-V 1)."""
+and the upstream outputs: stdout and every DP call's score (-V 1). Also writes
+tests/golden/mtdna_full.json.gz: the reference's data/mtDNA.fasta (393 reads) with the
+upstream outputs of the README's commands and of -r 0/1/2 and -B."""
 from __future__ import annotations
 
 import gzip
@@ -28,6 +30,19 @@ RUNS = [
     ("config2_S", "config2", ["-S", "-r", "1"]),
     ("config2", "config2", ["-r", "1"]),
     ("config4_S", "config4", ["-S", "-k", "15", "-w", "10", "-r", "0"]),
+    # the consensus of the two config-2 runs: the ungapped form of the MSA consensus row
+    ("config2_r0", "config2", ["-r", "0"]),
+    ("config2_S_r0", "config2", ["-S", "-r", "0"]),
+]
+# the reference's own data file, whole (393 reads), with the README's commands
+MTDNA = os.environ.get("MPOA_MTDNA", "/root/reference/data/mtDNA.fasta")
+MTDNA_RUNS = [
+    ("mtdna", ["-r", "0"]),
+    ("mtdna", ["-r", "1"]),
+    ("mtdna", ["-r", "2"]),
+    ("mtdna", ["-B", "-r", "0"]),
+    # README.md's MSA command: upstream dies on a sub-window that ends before it begins
+    ("mtdna", ["-S", "-t", "12", "-r", "1"]),
 ]
 
 
@@ -63,6 +78,22 @@ def main() -> None:
     with gzip.open(os.path.join(HERE, "full_configs.json.gz"), "wt") as f:
         json.dump({"generator": "tests/golden/make_full_configs.py",
                    "upstream": "minipoa (NCl3-lhd/POA)", "cases": out}, f)
+    with open(MTDNA) as f:
+        fa = f.read()
+    mt = []
+    for name, args in MTDNA_RUNS:
+        p = subprocess.run([REF_BIN, MTDNA, *args, "-V", "1"], capture_output=True, text=True)
+        scores = [int(l.split(":", 1)[1]) for l in p.stderr.splitlines()
+                  if l.startswith("score:")]
+        mt.append({"case": name, "args": args, "stdout": p.stdout, "scores": scores,
+                   "returncode": p.returncode})
+        print(f"{name:10s} {' '.join(args):30s} rc={p.returncode} calls={len(scores)} "
+              f"stdout={len(p.stdout)}B", flush=True)
+    with gzip.open(os.path.join(HERE, "mtdna_full.json.gz"), "wt") as f:
+        json.dump({"generator": "tests/golden/make_full_configs.py",
+                   "upstream": "minipoa (NCl3-lhd/POA)", "source": "data/mtDNA.fasta",
+                   "fasta": fa, "fasta_sha256": hashlib.sha256(fa.encode()).hexdigest(),
+                   "cases": mt}, f)
 
 
 if __name__ == "__main__":

@@ -30,10 +30,15 @@ def case_id(c: dict) -> str:
 
 def parse_args(args: list[str]) -> dict:
     """minipoa flags to parameter fields. Scores are taken as typed: initPara negates positive
-    penalties itself (src/handle_input.cpp:45-77), and so do both libraries."""
+    penalties itself (src/handle_input.cpp:45-77), and so do both libraries. -t (threads) is
+    accepted and dropped: it changes no output (src/align.cpp:1796-1823 gives the same
+    segments), and the device path has no host threads to set."""
     p: dict = {}
     i = 0
     while i < len(args):
+        if args[i] == "-t":
+            i += 2
+            continue
         field, val = _FLAGS[args[i]]
         if val:
             p[field] = int(args[i + 1])
@@ -63,3 +68,23 @@ def parse_fasta(text: str):
     off[1:] = np.cumsum([len(s) for s in joined])
     seq = np.frombuffer("".join(joined).encode(), np.uint8).copy()
     return seq, off, names, comments
+
+
+def check_msa_consensus_row(text3: str, text_r1: str, text_r0: str) -> None:
+    """The -r 3 output against upstream's -r 1 and -r 0 texts of the same run: the MSA rows are
+    exactly -r 1's; one more row follows, ">consensus sequence", whose bases without gaps are
+    -r 0's consensus and each of whose bases stands in a column where some read has the same
+    base (a consensus node lies on a read's path, and both sit in the node's column)."""
+    assert text3.startswith(text_r1)
+    tail = text3[len(text_r1):].split("\n")
+    assert len(tail) == 3 and tail[0] == ">consensus sequence" and tail[2] == ""
+    row = tail[1]
+    r0 = text_r0.split("\n")
+    assert r0[0] == ">consensus sequence"
+    assert row.replace("-", "") == r0[1]
+    rows = [ln for ln in text_r1.splitlines() if not ln.startswith(">")]
+    assert rows and all(len(r) == len(row) for r in rows)
+    for c, ch in enumerate(row):
+        if ch != "-":
+            assert any(r[c] == ch for r in rows), f"column {c}"
+
