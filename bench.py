@@ -101,6 +101,97 @@ def plain_form(ctx, params, batch, sample: int, gpu_texts, batched_wps: float, l
             "outputs_identical": all(texts[i] == gpu_texts[int(w)] for i, w in enumerate(idx))}
 
 
+SINGLE = [
+    # (name, generator, flags, workload): the single-window configurations, run through
+    # mpoa_align_reads (minipoa's whole main loop on one read set)
+    ("config1", "config1", ["-r", "0"],
+     "consensus: 20 reads of a 16,569 bp template at 1% sub, 0.5% ins, 0.5% del"),
+    ("config2", "config2", ["-r", "1"],
+     "row-column MSA: 100 reads of a 2,000 bp template at 10% error (4:3:3, geometric indels)"),
+    ("config2_S", "config2", ["-S", "-r", "1"], "the same reads, -S seeding (README form)"),
+    ("config4_S", "config4", ["-S", "-k", "15", "-w", "10", "-r", "0"],
+     "50 reads of a 20,000 bp template at 12% error (2:4:4), -S k=15 w=10, graph past 100k "
+     "nodes"),
+]
+
+
+def single_window(ctx, cores: int, with_cpu: bool, log) -> dict:
+    """The single-window configurations: device wall time of mpoa_align_reads (best of two
+    runs), GCUPS over band cells, algorithmic HBM rate over that wall time, and the upstream
+    binary on the same FASTA on the host cores (-t 1 without -S, where minipoa has no other
+    parallelism; -t cores with -S), outputs compared."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import golden_util as gu
+    from poa_amd import api, synth
+    out = {}
+    for name, gen, args, what in SINGLE:
+        reads = getattr(synth, gen)()
+        tmp = tempfile.mkdtemp(prefix="mpoa_single_")
+        try:
+            path = os.path.join(tmp, "r.fa")
+            reads.to_fasta(path)
+            with open(path) as f:
+                seq, off, names, comments = gu.parse_fasta(f.read())
+            params = api.Params(**gu.parse_args(args))
+            walls = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                text, stats = ctx.align_reads(params, seq, off, names, comments, with_stats=True)
+                walls.append(time.perf_counter() - t0)
+            wall = min(walls)
+            cells = sum(s.cells for s in stats)
+            ecells = sum(s.edge_cells for s in stats)
+            logical = sum(s.logical_cells for s in stats)
+            bpc = 4 * (3 + 2 * ecells / max(cells, 1))
+            ach = cells * bpc / wall / 1e9
+            rec = {"workload": what, "flags": " ".join(args), "dp_calls": len(stats),
+                   "band_cells": int(cells), "logical_cells": int(logical),
+                   "gpu_wall_s": round(wall, 4), "gcups": round(cells / wall / 1e9, 4),
+                   "roofline": {"bound": "hbm", "achieved": round(ach, 2),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(ach / HBM_PEAK_GBS, 6),
+                                "bytes_per_cell": round(bpc, 3),
+                                "note": "algorithmic bytes over the call's wall time"}}
+            if with_cpu and os.path.exists(REF_BIN):
+                threads = cores if "-S" in args else 1
+                t0 = time.perf_counter()
+                p = subprocess.run([REF_BIN, path, *args, "-t", str(threads)],
+                                   capture_output=True, text=True, check=True)
+                cw = time.perf_counter() - t0
+                rec["cpu_baseline"] = {"value": round(cw, 4), "unit": "s", "cores": threads,
+                                       "kind": "reference", "gcups": round(cells / cw / 1e9, 4),
+                                       "outputs_identical_to_gpu": p.stdout == text}
+                rec["gpu_over_cpu"] = round(cw / wall, 3)
+            log(f"single {name}: GPU {wall:.3f} s ({rec['gcups']} GCUPS)" +
+                (f", CPU {rec['cpu_baseline']['value']} s on {rec['cpu_baseline']['cores']} "
+                 f"cores, identical {rec['cpu_baseline']['outputs_identical_to_gpu']}"
+                 if "cpu_baseline" in rec else ""))
+            out[name] = rec
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
+def estimate_form(ctx, params, batch, steps: int, log) -> dict:
+    """The same batch scheduled by the read-length estimate (no network): windows/s, device
+    memory of the arenas and overflow re-runs, for the A/B against the learned schedule."""
+    from poa_amd import api
+    b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
+    b.execute()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.execute()
+    wall = time.perf_counter() - t0
+    st = b.stats()
+    texts = b.results().texts
+    b.free()
+    wps = batch.n_windows * steps / wall
+    log(f"read-length estimate: {wps:.1f} windows/s, pools {st.pool_bytes / 1e9:.1f} GB, "
+        f"{st.reruns} re-runs")
+    return {"windows_per_s": round(wps, 1), "pool_bytes": int(st.pool_bytes),
+            "reruns": int(st.reruns), "steps": steps, "texts": texts}
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on this workload, from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration);
@@ -151,6 +242,10 @@ def main() -> int:
                     help="enqueue the launches one by one instead of replaying the hipGraph")
     ap.add_argument("--plain-sample", type=int, default=100,
                     help="windows run in the plain one-window-per-launch form (0: skip)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-window configurations")
+    ap.add_argument("--no-ab", action="store_true",
+                    help="skip the learned-against-estimate scheduling A/B")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (torch), the known-size "
                          "kernel that scales rocprof FETCH_SIZE/WRITE_SIZE to bytes")
@@ -285,6 +380,7 @@ def main() -> int:
             "dp_calls": int(st.dp_calls),
             "band_cells": int(st.cells),
             "edge_cells_per_cell": round(ebar, 4),
+            "logical_cells": int(st.logical_cells),
             "reruns": int(st.reruns),
             "launches": int(st.launches),
             "slots": int(st.slots),
@@ -324,10 +420,20 @@ def main() -> int:
     if args.plain_sample:
         line["plain_one_window_per_launch"] = plain_form(ctx, params, batch, args.plain_sample,
                                                          res.texts, windows_per_s, log)
+    cores = args.cpu_cores or min(16, os.cpu_count() or 1)
     if not args.no_cpu:
-        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(batch, res.texts, args.cpu_sample, cores)
+    pool_learned = int(st.pool_bytes)
     b.free()
+    if learned and not args.no_ab:
+        est = estimate_form(ctx, params, batch, 2, log)
+        line["scheduling_ab"] = {
+            "learned": {"windows_per_s": round(windows_per_s, 1), "pool_bytes": pool_learned,
+                        "reruns": int(st.reruns)},
+            "read_length_estimate": {k: v for k, v in est.items() if k != "texts"},
+            "outputs_identical": est["texts"] == res.texts}
+    if not args.no_single:
+        line["single_window"] = single_window(ctx, cores, not args.no_cpu, log)
     ctx.close()
     print(json.dumps(line), flush=True)
     return 0

@@ -56,6 +56,8 @@ struct WinShared {
   int msa_cols;
   int64_t cells, edge_cells;
   int64_t call_cells, call_ecells;  // of the last DP call
+  int64_t call_logical;             // its logical cells, sum over rows 1..n-1 of Me-Ms+1
+  int64_t logical;                  // the window's
   int call_width;                   // widest row of the last DP call
   int bt_rc;                        // backtrack result of the last DP call
   bool bt_serial;                   // it ran on one lane
@@ -1100,9 +1102,13 @@ __device__ __forceinline__ bool plan_static(const Arena& A, WinShared& S, const 
         const int Ms = max(0, min(lo, hi) - w), Me = min(m, max(lo, hi) + w);
         bs = Ms / Q;
         be = Me / Q + 2;
+        if (r > 0) atomicAdd(reinterpret_cast<unsigned long long*>(&S.call_logical),
+                             (unsigned long long)(Me - Ms + 1));
       } else {
         bs = 0;
         be = m / Q + 2;
+        if (r > 0) atomicAdd(reinterpret_cast<unsigned long long*>(&S.call_logical),
+                             (unsigned long long)(m + 1));
       }
       A.Bs[r] = bs;
       A.Be[r] = be;
@@ -1284,6 +1290,81 @@ __device__ __forceinline__ int bt_pick(bool cand, int w, int lane) {
   return best < 0 ? -1 : 63 - (best & 63);
 }
 
+// A run of match moves along consecutive ranks, checked by the whole wave at once: lane t
+// takes the move from (row i - t, column acj - t) to (i - t - 1, acj - t - 1) and checks that
+// minipoa's first rule (the match-only step, align.cpp:1591-1626) takes exactly it: the row's
+// heaviest in-window predecessor (first of the largest in-weight) is row i - t - 1, its base is
+// the query's, its in-weight passes the ind/10 rule, the column is inside the row's band and
+// the predecessor's, and H equals the predecessor's diagonal H plus the match score. The
+// diagonal cell is the one lane t + 1 loads for its own move. Lane 0 also needs op to hold M.
+// Lanes 0..K (K <= 63) take part. Moves of the leading lanes that pass are taken (at most K
+// per call, two round trips), and logged as their predecessor's local rank. Returns their number; *nbs, *nbe, *noff, *nrr get
+// the geometry and record of the row reached.
+__device__ __forceinline__ int bt_run(const Arena& A, const WinShared& S, int match, int i,
+                                      int acj, int op, int K, int* log, int* nbs, int* nbe,
+                                      int* noff, RowRec* nrr) {
+  const int lane = threadIdx.x & 63;
+  const int beg = S.beg_i;
+  const int r = i - lane, a = acj - lane;
+  const bool live = lane <= K && r >= 0 && a >= 0;
+  int bs = 0, be = 0, off = 0, qc = -1;
+  RowRec rec{};
+  if (live) {
+    bs = A.Bs[r];
+    be = A.Be[r];
+    off = (int)A.row_off[r];
+    rec = load_rowrec(A.rowrec, beg + r);
+    if (a > 0) qc = A.qseq[a - 1];
+  }
+  const bool in_band = live && a >= bs * Q && a <= (be - 1) * Q;
+  int h = 0;
+  if (in_band) h = A.H[off + a - bs * Q];
+  const int up_bs = __shfl_down(bs, 1, 64), up_be = __shfl_down(be, 1, 64);
+  const int up_h = __shfl_down(h, 1, 64);
+  bool ok = in_band && r >= 1 && a >= 1 && lane < K && rec.np >= 1 && rec.np <= 3;
+  if (ok) {
+    int kh = -1, wmax = -1, ph = -1;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int prank = k == 0 ? rec.p0 : k == 1 ? rec.p1 : rec.p2;
+      const int w = k == 0 ? rec.w0 : k == 1 ? rec.w1 : rec.w2;
+      if (k < rec.np && prank >= beg && w > wmax) {
+        wmax = w;
+        kh = k;
+        ph = prank - beg;
+      }
+    }
+    ok = kh >= 0 && ph == r - 1;
+    if (ok) {
+      const int pbase = ph == 0 ? CODE_N : (rec.pk >> (3 * kh)) & 7;
+      const int rule = (rec.pk >> (9 + kh)) & 1;
+      const int pj = a - up_bs * Q;
+      ok = pbase == qc && rule && pj - 1 >= 0 && pj - 1 < (up_be - up_bs - 1) * Q &&
+           h == up_h + match;
+    }
+  }
+  if (lane == 0 && !(op & M_OP)) ok = false;
+  const uint64_t good = __ballot(ok);
+  const int L = (int)__builtin_ctzll(~good);
+  if (lane < L) log[a - 1] = r - 1;
+  *nbs = __builtin_amdgcn_readlane(bs, L);
+  *nbe = __builtin_amdgcn_readlane(be, L);
+  *noff = __builtin_amdgcn_readlane(off, L);
+  nrr->row_off = __builtin_amdgcn_readlane(rec.row_off, L);
+  nrr->bsbe = __builtin_amdgcn_readlane(rec.bsbe, L);
+  nrr->k0 = __builtin_amdgcn_readlane(rec.k0, L);
+  nrr->np = __builtin_amdgcn_readlane(rec.np, L);
+  nrr->p0 = __builtin_amdgcn_readlane(rec.p0, L);
+  nrr->p1 = __builtin_amdgcn_readlane(rec.p1, L);
+  nrr->p2 = __builtin_amdgcn_readlane(rec.p2, L);
+  nrr->pk = __builtin_amdgcn_readlane(rec.pk, L);
+  nrr->w0 = __builtin_amdgcn_readlane(rec.w0, L);
+  nrr->w1 = __builtin_amdgcn_readlane(rec.w1, L);
+  nrr->w2 = __builtin_amdgcn_readlane(rec.w2, L);
+  nrr->last = __builtin_amdgcn_readlane(rec.last, L);
+  return L;
+}
+
 __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc,
                                               int* log) {
   const int lane = threadIdx.x & 63;
@@ -1297,7 +1378,32 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
   int p = -1, pw = 0, pbase = 0, prule = 0, pbs = 0, pbe = 0, poff = 0;
   RowRec prr{};
   bool setup = true;
+  // Runs of match moves along consecutive ranks (bt_run) are tried after two match moves in a
+  // row to the previous rank (where a run is likely to go on), and again after a run that used
+  // its whole width; the width starts at 8 moves and doubles after such a run (up to 63), since
+  // a run's loads cost more the more lanes take part.
+  bool run = false;
+  int width = 8, streak = 0;
   for (;;) {
+    if (run && (op & M_OP) && acj > 0 && i > 0) {
+      int nbs, nbe, noff;
+      RowRec nrr;
+      const int L = bt_run(A, S, match, i, acj, op, width, log, &nbs, &nbe, &noff, &nrr);
+      run = L == width;  // a full run may go on; otherwise the next move is not such a match
+      width = run ? min(2 * width, 63) : 8;
+      if (!run) streak = 0;
+      if (L > 0) {
+        i -= L;
+        acj -= L;
+        op = ALL_OP;
+        bs_i = nbs;
+        be_i = nbe;
+        off_i = noff;
+        rr = nrr;
+        setup = true;
+        continue;
+      }
+    }
     if (setup) {
       if (rr.np > 63) return 1;
       p = -1;
@@ -1358,11 +1464,13 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
     // kind: 1 substitution move (logged), 2 deletion move, 3 insertion (logged)
     int k = -1, kind = 0, nop = op;
     // 1. match-only move with the in-weight rule (align.cpp:1591-1626)
+    bool matched = false;
     if ((op & M_OP) && acj > 0) {
       k = bt_pick(m_in && pbase == qc && h == hdiag + match, pw, lane);
       if (k >= 0 && __builtin_amdgcn_readlane(prule, k)) {
         kind = 1;
         nop = ALL_OP;
+        matched = true;
       }
     }
     // 2. deletion: open first, then extend (align.cpp:1627-1667)
@@ -1394,6 +1502,8 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
       }
     }
     if (kind == 0) return MPOA_E_BACKTRACK;
+    streak = matched && __builtin_amdgcn_readlane(p, k) == i - 1 ? streak + 1 : 0;
+    run = streak >= 2;
     if (kind != 2) {
       if (lane == 0) log[acj - 1] = kind == 1 ? __builtin_amdgcn_readlane(p, k) : -1;
       acj--;
@@ -1464,6 +1574,7 @@ __device__ __forceinline__ void adaptive_band_row(const Arena& A, WinShared& S, 
   const int Me = min(m, max(max(A.Pr[i], lo + A.Or[i]), hi) + w + ow);
   A.Bs[i] = Ms / Q;
   A.Be[i] = Me / Q + 2;
+  if (i > 0) S.call_logical += Me - Ms + 1;
 }
 
 // Adaptive-band update after row i (src/align.cpp:1516-1568); one lane, argmax given.
@@ -1543,6 +1654,7 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
   if (sc.f) w += m / sc.f;
   const int64_t t0 = now();
   const int full_len = (m / Q + 2) * Q;
+  if (tid == 0) S.call_logical = 0;
   if (!adaptive) {
     if (tid == 0) S.row_wmax = 0;
     __syncthreads();
@@ -1716,6 +1828,7 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     S.edge_cells += ecells;
     S.call_cells = cells;
     S.call_ecells = ecells;
+    S.logical += S.call_logical;
     S.call_width = wmax;
     S.score = A.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)];
   }
@@ -2169,6 +2282,7 @@ __device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, Wi
     res.out_len = S.status == MPOA_OK ? S.cons_len : 0;
     res.cells = S.cells;
     res.edge_cells = S.edge_cells;
+    res.logical = S.logical;
     res.nodes = S.n_nodes;
     res.edges = S.n_in_e;
   }
@@ -2181,7 +2295,7 @@ __device__ __forceinline__ void window_begin(const KernelArgs& a, Arena& A, WinS
     a.result[w].t_begin = now();
     S.status = MPOA_OK;
     S.n_nodes = 0; S.n_in_e = 0; S.n_out_e = 0;
-    S.cells = 0; S.edge_cells = 0;
+    S.cells = 0; S.edge_cells = 0; S.logical = 0;
     S.cons_len = 0;
     for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
     new_node(A, S, a.caps, CODE_N);  // source 0 (graph.cpp:129-135)
@@ -2229,6 +2343,7 @@ __device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinSha
         cs.width = S.call_width;
         cs.cells = S.call_cells;
         cs.edge_cells = S.call_ecells;
+        cs.logical = S.call_logical;
         cs.nodes = S.n_nodes;
         cs.edges = S.n_in_e;
       }
@@ -2258,13 +2373,13 @@ __device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinSha
 __device__ void state_load(WinShared& S, const WinState& st) {
   S.n_nodes = st.n_nodes; S.n_in_e = st.n_in_e; S.n_out_e = st.n_out_e;
   S.status = st.status; S.msa_cols = st.msa_cols; S.cons_len = st.cons_len;
-  S.cells = st.cells; S.edge_cells = st.edge_cells;
+  S.cells = st.cells; S.edge_cells = st.edge_cells; S.logical = st.logical;
   for (int k = 0; k < PH_N; k++) S.ticks[k] = st.ticks[k];
 }
 __device__ void state_store(const WinShared& S, WinState& st) {
   st.n_nodes = S.n_nodes; st.n_in_e = S.n_in_e; st.n_out_e = S.n_out_e;
   st.status = S.status; st.msa_cols = S.msa_cols; st.cons_len = S.cons_len;
-  st.cells = S.cells; st.edge_cells = S.edge_cells;
+  st.cells = S.cells; st.edge_cells = S.edge_cells; st.logical = S.logical;
   for (int k = 0; k < PH_N; k++) st.ticks[k] = S.ticks[k];
 }
 
@@ -2306,7 +2421,7 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
     if (tid == 0) {
       out.status = S.n_rows <= 0 ? MPOA_E_EMPTY_WINDOW : MPOA_E_OVERFLOW;
       out.score = 0; out.rows = S.n_rows; out.qlen = d.qlen;
-      out.cells = 0; out.edge_cells = 0; out.width = 0;
+      out.cells = 0; out.edge_cells = 0; out.width = 0; out.logical = 0;
     }
     return;
   }
@@ -2334,6 +2449,7 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
     out.width = S.call_width;
     out.cells = S.call_cells;
     out.edge_cells = S.call_ecells;
+    out.logical = S.call_logical;
     for (int k = 0; k < PH_N; k++) out.ticks[k] = S.ticks[k];
   }
 }
@@ -2372,6 +2488,7 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
           if (g.status != MPOA_OK && S.status == MPOA_OK) S.status = g.status;
           S.cells += g.cells;
           S.edge_cells += g.edge_cells;
+          S.logical += g.logical;
           for (int t = 0; t < PH_N; t++) S.ticks[t] += g.ticks[t];
         }
       }
@@ -2463,6 +2580,7 @@ __global__ __launch_bounds__(NT) void k_graph_op(GraphOpArgs g) {
       o.width = S.call_width;
       o.cells = S.call_cells;
       o.edge_cells = S.call_ecells;
+      o.logical = S.call_logical;
     }
     return;
   }
