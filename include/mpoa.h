@@ -69,6 +69,8 @@ typedef struct mpoa_call_stat {
   int32_t nodes;        /* graph nodes and edges when the call ran */
   int32_t edges;
   int32_t window;       /* window index (batch traces) */
+  int64_t logical_cells; /* sum over rows 1..n-1 of Me - Ms + 1, the band before the 8-column
+                            quantum (align.cpp:1383-1394) */
 } mpoa_call_stat;
 
 /* minipoa's whole main loop over one read set: for each read in input order,
@@ -218,6 +220,7 @@ typedef struct mpoa_batch_stats {
   int32_t graph_launches; /* graph replays in the last execute (0: eager launches) */
   int32_t graph_builds;   /* graphs built over the batch's life (a rebuild follows any change
                              of a launch's arguments) */
+  int64_t logical_cells;  /* sum over all DP rows of Me - Ms + 1 (cells counts 8*block_num) */
 } mpoa_batch_stats;
 
 /* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window
@@ -255,6 +258,7 @@ typedef struct mpoa_window_stat {
   int64_t cells, edge_cells;
   int64_t ticks[8];
   int64_t t_begin, t_end;                /* device wall clock at the window's start and end */
+  int64_t logical_cells;
 } mpoa_window_stat;
 int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);
 

@@ -69,14 +69,16 @@ class CallStat(ctypes.Structure):
     _fields_ = [("read", ctypes.c_int32), ("rows", ctypes.c_int32), ("qlen", ctypes.c_int32),
                 ("score", ctypes.c_int32), ("cells", ctypes.c_int64),
                 ("edge_cells", ctypes.c_int64), ("width", ctypes.c_int32),
-                ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32), ("window", ctypes.c_int32)]
+                ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32), ("window", ctypes.c_int32),
+                ("logical_cells", ctypes.c_int64)]
 
 
 class WindowStat(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("nodes", ctypes.c_int32), ("edges", ctypes.c_int32),
                 ("launch", ctypes.c_int32), ("cells", ctypes.c_int64),
                 ("edge_cells", ctypes.c_int64), ("ticks", ctypes.c_int64 * 8),
-                ("t_begin", ctypes.c_int64), ("t_end", ctypes.c_int64)]
+                ("t_begin", ctypes.c_int64), ("t_end", ctypes.c_int64),
+                ("logical_cells", ctypes.c_int64)]
 
 
 class LaunchStat(ctypes.Structure):
@@ -91,7 +93,8 @@ class BatchStats(ctypes.Structure):
                 ("reruns", ctypes.c_int64), ("device_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("pool_bytes", ctypes.c_int64),
-                ("graph_launches", ctypes.c_int32), ("graph_builds", ctypes.c_int32)]
+                ("graph_launches", ctypes.c_int32), ("graph_builds", ctypes.c_int32),
+                ("logical_cells", ctypes.c_int64)]
 
 
 _lib = None

@@ -48,6 +48,7 @@ mpoa_call_stat call_stat_of(const CallStat& c, int read, int window) {
   o.score = c.score;
   o.cells = c.cells;
   o.edge_cells = c.edge_cells;
+  o.logical_cells = c.logical;
   o.width = c.width;
   o.nodes = c.nodes;
   o.edges = c.edges;
@@ -644,6 +645,7 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
         c.score = g.score;
         c.cells = g.cells;
         c.edge_cells = g.edge_cells;
+        c.logical_cells = g.logical;
         c.width = g.width;
         c.nodes = st.n_nodes;
         c.edges = st.n_in_e;
@@ -1151,6 +1153,7 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
     for (int w = 0; w < n && w < (int)b->res.size(); w++) {
       stats->cells += b->res[w].cells;
       stats->edge_cells += b->res[w].edge_cells;
+      stats->logical_cells += b->res[w].logical;
       stats->dp_calls += std::max(0, b->info[w].n_reads - 1);
     }
     stats->reruns = b->reruns;
@@ -1224,6 +1227,7 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
     o.t_begin = r.t_begin;
     o.t_end = r.t_end;
     o.cells = r.cells;
+    o.logical_cells = r.logical;
     o.edge_cells = r.edge_cells;
     for (int k = 0; k < 8; k++) o.ticks[k] = r.ticks[k];
   }

@@ -127,6 +127,7 @@ struct WinResult {
   int status;
   int out_len;     // consensus length, or MSA columns
   int64_t cells, edge_cells;
+  int64_t logical;  // sum over DP rows 1..n-1 of Me-Ms+1 (the unquantised band)
   int nodes, edges;
   int64_t ticks[PH_N];
   int64_t t_begin, t_end;  // device wall clock at the window's start and end
@@ -135,7 +136,7 @@ struct WinResult {
 // One DP call (one read against the graph of the reads before it).
 struct CallStat {
   int score, rows, qlen, width;  // width: widest row, 8*block_num
-  int64_t cells, edge_cells;
+  int64_t cells, edge_cells, logical;
   int nodes, edges;              // graph size when the call ran
 };
 
@@ -167,14 +168,14 @@ struct SegDesc {
 
 struct SegResult {
   int status, score, rows, qlen, width, pad;
-  int64_t cells, edge_cells;
+  int64_t cells, edge_cells, logical;
   int64_t ticks[PH_N];
 };
 
 // Window state kept in device memory between the launches of a seeded window.
 struct WinState {
   int n_nodes, n_in_e, n_out_e, status, msa_cols, cons_len, pad0, pad1;
-  int64_t cells, edge_cells;
+  int64_t cells, edge_cells, logical;
   int64_t ticks[PH_N];
 };
 
@@ -200,7 +201,7 @@ enum GraphOp { GOP_INIT = 0, GOP_POA = 1, GOP_ADD_PATH = 2, GOP_TOPSORT = 3, GOP
 
 struct GraphOpOut {
   int status, score, rows, qlen, width, pad;
-  int64_t cells, edge_cells;
+  int64_t cells, edge_cells, logical;
 };
 
 struct GraphOpArgs {

@@ -122,3 +122,27 @@ def test_device_msa_consensus_row_batched_and_calls(ctx):
         g.topsort(1 if i + 1 == n else 0)
     assert g.output(3) == ctx.align_reads(params, r.seq, r.off)
     g.close()
+
+
+LOGICAL_CASES = [c for c in gu.load_cases()
+                 if (c["case"], tuple(c["args"])) in {
+                     ("mtdna40", ("-r", "0")), ("mtdna40", ("-B", "-r", "0")),
+                     ("mtdna40", ("-f", "0", "-r", "0")), ("cfg2_seeded", ("-S", "-r", "1")),
+                     ("cfg4_small", ("-S", "-k", "15", "-w", "10", "-r", "0"))}]
+
+
+@pytest.mark.parametrize("case", LOGICAL_CASES, ids=gu.case_id)
+def test_device_logical_cells(ctx, case):
+    """Every DP call's logical cells (rows 1..n-1 of Me - Ms + 1) equal the oracle's, in the
+    static, full-width, adaptive and seeded bands; the batch total is their sum."""
+    seq, off, names, comments = gu.parse_fasta(case["fasta"])
+    p = gu.parse_args(case["args"])
+    _, stats = ctx.align_reads(api.Params(**p), seq, off, names, comments, with_stats=True)
+    _, want = oracle_lib.window(p, seq, off, names, comments)
+    assert [s.logical_cells for s in stats] == [w.logical_cells for w in want]
+    assert [s.cells for s in stats] == [w.cells for w in want]
+    if not p.get("seeding"):
+        b = api.Batch(ctx, api.Params(**p), np.array([0, len(off) - 1], np.int32), seq, off)
+        b.execute()
+        assert b.stats().logical_cells == sum(w.logical_cells for w in want)
+        b.free()
