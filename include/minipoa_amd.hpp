@@ -86,6 +86,7 @@ struct para_t {
     p.mm_w = mm_w;
     p.poa_w = poa_w;
     p.result = result;
+    p.progressive = progressive_poa;
     return p;
   }
 };
@@ -137,10 +138,11 @@ inline void lists(const std::vector<seq_t>& seqs, std::string* name, std::vector
 }
 }  // namespace detail
 
-// minimizer_t (include/minimizer.h:9-32): the read set's minimizers for -S seeding.
+// minimizer_t (include/minimizer.h:9-32): the read set's minimizers for -S seeding, and the
+// read order (input order until get_guide_tree).
 struct minimizer_t {
   mpoa_seeder* h = nullptr;
-  std::vector<int> ord, rid_to_ord;  // input order (no progressive ordering)
+  std::vector<int> ord, rid_to_ord;
   minimizer_t(para_t* para, const std::vector<seq_t>& seqs) {
     std::string all;
     std::vector<int64_t> off(1, 0);
@@ -156,6 +158,17 @@ struct minimizer_t {
     }
   }
   ~minimizer_t() { mpoa_seeder_destroy(h); }
+  // minimizer_t::get_guide_tree (minimizer.cpp:195-386): with -p, the order reads are added in
+  void get_guide_tree(para_t* para) {
+    if (!para->progressive_poa) return;
+    const int32_t n = mpoa_seeder_order(h, nullptr, 0);
+    std::vector<int32_t> o((size_t)(n > 0 ? n : 1));
+    mpoa_seeder_order(h, o.data(), n);
+    for (int i = 0; i < n; i++) {
+      ord[i] = o[i];
+      rid_to_ord[o[i]] = i;
+    }
+  }
   minimizer_t(const minimizer_t&) = delete;
   minimizer_t& operator=(const minimizer_t&) = delete;
 };
@@ -186,7 +199,8 @@ struct graph {
   void topsort(const para_t* /*para*/, int op) { check(mpoa_graph_topsort(h, op)); }
 
   // The outputs as text, and printed to std::cout as minipoa prints them (graph.cpp:237-433).
-  std::string text(int result, const std::vector<seq_t>* seqs = nullptr) {
+  std::string text(int result, const std::vector<seq_t>* seqs = nullptr,
+                   const std::vector<int>* rid_to_ord = nullptr) {
     std::string name, comment;
     std::vector<int64_t> name_off, comment_off;
     if (seqs) detail::lists(*seqs, &name, &name_off, &comment, &comment_off);
@@ -195,18 +209,25 @@ struct graph {
     const int64_t* no = seqs ? name_off.data() : nullptr;
     const char* cm = seqs ? comment.data() : nullptr;
     const int64_t* co = seqs ? comment_off.data() : nullptr;
-    check(mpoa_graph_output(h, result, nm, no, cm, co, nullptr, 0, &n));
+    std::vector<int32_t> map;
+    if (rid_to_ord) {
+      bool identity = true;
+      for (size_t i = 0; i < rid_to_ord->size(); i++) identity &= (*rid_to_ord)[i] == (int)i;
+      if (!identity) map.assign(rid_to_ord->begin(), rid_to_ord->end());
+    }
+    const int32_t* ro = map.empty() ? nullptr : map.data();
+    check(mpoa_graph_output(h, result, nm, no, cm, co, ro, nullptr, 0, &n));
     std::string out((size_t)n, '\0');
-    check(mpoa_graph_output(h, result, nm, no, cm, co, &out[0], n, &n));
+    check(mpoa_graph_output(h, result, nm, no, cm, co, ro, &out[0], n, &n));
     return out;
   }
   void output_consensus(bool /*needCoverages*/ = false) { std::cout << text(0); }
-  void output_rc_msa(para_t* /*para*/, const std::vector<int>& /*rid_to_ord*/,
+  void output_rc_msa(para_t* /*para*/, const std::vector<int>& rid_to_ord,
                      const std::vector<seq_t>& seqs) {
-    std::cout << text(1, &seqs);
+    std::cout << text(1, &seqs, &rid_to_ord);
   }
-  void output_gfa(const std::vector<int>& /*rid_to_ord*/, const std::vector<seq_t>& seqs) {
-    std::cout << text(2, &seqs);
+  void output_gfa(const std::vector<int>& rid_to_ord, const std::vector<seq_t>& seqs) {
+    std::cout << text(2, &seqs, &rid_to_ord);
   }
 };
 

@@ -30,6 +30,11 @@ typedef struct mpoa_params {
                                                  3 (no minipoa counterpart): the -r 1 MSA,
                                                  then a consensus row, ">consensus sequence",
                                                  each consensus base in its node's column */
+  int32_t progressive;                        /* -p: reads added in minipoa's guide-tree order
+                                                 (minimizer.cpp:195-386); outputs keep input
+                                                 order. Not with result 2: minipoa's GFA writer
+                                                 indexes its paths past their ends then
+                                                 (graph.cpp:369,423) */
 } mpoa_params;
 
 /* Error codes. minipoa terminates the process in these cases; the library returns. */
@@ -133,6 +138,10 @@ typedef struct mpoa_seeder mpoa_seeder;
 int mpoa_seeder_create(const mpoa_params* params, int32_t n_reads, const char* seq,
                        const int64_t* seq_off, mpoa_seeder** out);
 void mpoa_seeder_destroy(mpoa_seeder* s);
+/* minimizer_t::get_guide_tree (minimizer.cpp:195-386) and its ord: ord[i] is the read added
+ * i-th, minipoa's guide-tree order when params->progressive was set, else input order. Writes
+ * up to cap entries; returns the number of reads. */
+int32_t mpoa_seeder_order(mpoa_seeder* s, int32_t* ord, int32_t cap);
 
 /* alignment(para, DAG, mm, rid, qseq, qlen, mpool) (include/align.h:16,
  * src/align.cpp:1741-1828): the trivial path of a graph without bases, the whole-graph poa(),
@@ -154,14 +163,15 @@ int mpoa_graph_add_path(mpoa_graph* g, int32_t seq_id, const mpoa_res* res, int6
 int mpoa_graph_topsort(mpoa_graph* g, int32_t op);
 
 /* graph::output_consensus (result 0), output_rc_msa (1) or output_gfa (2), or the MSA with a
- * consensus row (3)
+ * consensus row (3); rid_to_ord (optional, as output_rc_msa's): read i prints the MSA row of
+ * the read added at position rid_to_ord[i] (not with GFA)
  * (src/graph.cpp:237-433, called at src/main.cpp:124-126) over the reads added so far, as the
  * text minipoa prints; names / comments as for mpoa_align_reads. The MSA of a graph not sorted
  * since its last add_path is taken after an op 1 sort, as output_rc_msa does. *out_len
  * receives the size (also with MPOA_E_OUT_CAP); out NULL sizes only. */
 int mpoa_graph_output(mpoa_graph* g, int32_t result, const char* name, const int64_t* name_off,
-                      const char* comment, const int64_t* comment_off, char* out, int64_t out_cap,
-                      int64_t* out_len);
+                      const char* comment, const int64_t* comment_off, const int32_t* rid_to_ord,
+                      char* out, int64_t out_cap, int64_t* out_len);
 
 /* The graph as host arrays (node_t fields, include/node.h:5-68): node count and in-edge count,
  * then per node id its base, par_id and aligned_node[5]; in- and out-edge CSR in list order

@@ -44,7 +44,7 @@ static inline char char_of(uint8_t code) { return "ACGTN-"[code < 6 ? code : 4];
 
 struct Params {
   int match, mismatch, gap_open, gap_ext, b, f, k, mm_w, poa_w, bw, result;
-  bool ab_band, seeding;
+  bool ab_band, seeding, progressive;
 };
 
 // initPara (handle_input.cpp:45-77): mismatch/gap scores forced non-positive; the
@@ -64,6 +64,7 @@ static Params normalise(const orc_params& p) {
   P.poa_w = p.poa_w;
   P.bw = 1000;  // main.cpp:65
   P.result = p.result;
+  P.progressive = p.progressive != 0;
   return P;
 }
 
@@ -1089,6 +1090,87 @@ struct Seeder {
   }
 };
 
+// minimizer_t::get_guide_tree (minimizer.cpp:195-386): the order reads are added in with -p.
+// Minimizer hits between every two reads (the smaller count per shared minimizer, summed) over
+// all minimizers sorted by hash; each read's own count; the Jaccard similarity of every ordered
+// pair weighted by 0.51 and 0.49 of the two lengths over the longest; the pair of the largest
+// similarity (first in row-major order) opens the order, then each next read is the one whose
+// similarities from the reads already placed, summed in their order, are largest (first wins).
+static std::vector<int> guide_tree(const Params& P, int n, const char* seq, const int64_t* off) {
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; i++) ord[i] = i;
+  if (n < 2) return ord;
+  std::vector<mm128> mm;
+  std::vector<int> len(n);
+  int maxl = 0;
+  for (int i = 0; i < n; i++) {
+    len[i] = (int)(off[i + 1] - off[i]);
+    maxl = std::max(maxl, len[i]);
+    mm_sketch(seq + off[i], len[i], P.mm_w, P.k, (uint32_t)i, mm);
+  }
+  if (mm.empty()) return ord;
+  radix_sort(mm.data(), mm.data() + mm.size());
+  // hit[i*(i+1)/2 + j], j <= i: shared minimizers of reads i and j; j == i: read i's total
+  std::vector<long> hit((size_t)n * (n + 1) / 2, 0);
+  std::vector<int> cnt(n);
+  size_t g = 0;
+  for (size_t t = 1; t <= mm.size(); t++) {
+    if (t < mm.size() && mm[t].x == mm[g].x) continue;
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (size_t u = g; u < t; u++) {
+      const int r = (int)(mm[u].y >> 32);
+      cnt[r]++;
+      hit[(size_t)r * (r + 1) / 2 + r]++;
+    }
+    for (int r1 = 0; r1 < n - 1; r1++)
+      for (int r2 = r1 + 1; r2 < n; r2++) hit[(size_t)r2 * (r2 + 1) / 2 + r1] += std::min(cnt[r1], cnt[r2]);
+    g = t;
+  }
+  std::vector<double> jac((size_t)n * n, 0.0);
+  double best = -1.0;
+  int bi = -1, bj = -1;
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) {
+      if (i == j) continue;
+      const long sh = i > j ? hit[(size_t)i * (i + 1) / 2 + j] : hit[(size_t)j * (j + 1) / 2 + i];
+      const long tot = hit[(size_t)i * (i + 1) / 2 + i] + hit[(size_t)j * (j + 1) / 2 + j] - sh;
+      double v = 0;
+      if (tot > 0) {
+        v = (0.0 + sh) / tot;
+        v = v * ((0.51 * len[i] / maxl) + (0.49 * len[j] / maxl));
+      }
+      jac[(size_t)i * n + j] = v;
+      if (v > best) {
+        best = v;
+        bi = i;
+        bj = j;
+      }
+    }
+  }
+  ord[0] = bi;
+  ord[1] = bj;
+  for (int in = 2; in < n; in++) {
+    double top = -1.0;
+    int pick = n;
+    for (int r2 = 0; r2 < n; r2++) {
+      double v = 0.0;
+      for (int t = 0; t < in; t++) {
+        if (ord[t] == r2) {
+          v = -1.0;
+          break;
+        }
+        v += jac[(size_t)ord[t] * n + r2];
+      }
+      if (v > top) {
+        top = v;
+        pick = r2;
+      }
+    }
+    ord[in] = pick;
+  }
+  return ord;
+}
+
 // alignment() (align.cpp:1741-1828). The serial (-t 1) and OpenMP forms give the same
 // segments (q_span is always k); the serial form is followed.
 static int alignment(const Params& P, Graph& G, Seeder* S, int rid, const char* qseq, int qlen,
@@ -1146,7 +1228,8 @@ struct Text {
 
 // output_rc_msa (graph.cpp:237-251)
 static void output_msa(Graph& G, const Params& P, int n_reads, const std::vector<std::string>& name,
-                       const std::vector<std::string>& comment, Text& o) {
+                       const std::vector<std::string>& comment, Text& o,
+                       const std::vector<int>* rid_to_ord = nullptr) {
   if (!G.is_topsorted) G.topsort(P, 1);
   const size_t L = G.rank.size() >= 2 ? G.rank.size() - 2 : 0;
   std::vector<std::string> rows(n_reads, std::string(L, '-'));
@@ -1166,7 +1249,8 @@ static void output_msa(Graph& G, const Params& P, int n_reads, const std::vector
       o.put(comment.empty() ? std::string() : comment[i]);
       o.put('\n');
     }
-    o.put(rows[i]);
+    // read i's row is that of the read added at position rid_to_ord[i] (graph.cpp:249)
+    o.put(rows[rid_to_ord ? (*rid_to_ord)[i] : i]);
     o.put('\n');
   }
 }
@@ -1177,8 +1261,9 @@ static void output_msa(Graph& G, const Params& P, int n_reads, const std::vector
 // such output; its rows are the -r 1 rows and its ungapped row the -r 0 consensus.
 static void output_msa_cons(Graph& G, const Params& P, int n_reads,
                             const std::vector<std::string>& name,
-                            const std::vector<std::string>& comment, Text& o) {
-  output_msa(G, P, n_reads, name, comment, o);
+                            const std::vector<std::string>& comment, Text& o,
+                            const std::vector<int>* rid_to_ord = nullptr) {
+  output_msa(G, P, n_reads, name, comment, o, rid_to_ord);
   const size_t L = G.rank.size() >= 2 ? G.rank.size() - 2 : 0;
   std::string row(L, '-');
   G.build_consensus();
@@ -1259,12 +1344,23 @@ static int run_window(const Params& P, int n_reads, const char* seq, const int64
   G.init();
   Seeder S;
   if (P.seeding) S.init(P, n_reads, seq, off);
+  // -p: reads in guide-tree order, read ord[i] added as the i-th (main.cpp:100-116)
+  if (P.progressive && P.result == 2) return ORC_E_ARG;
+  const std::vector<int> ord = P.progressive ? guide_tree(P, n_reads, seq, off) : [&] {
+    std::vector<int> id(n_reads);
+    for (int i = 0; i < n_reads; i++) id[i] = i;
+    return id;
+  }();
+  std::vector<int> rid_to_ord(n_reads);
+  for (int i = 0; i < n_reads; i++) rid_to_ord[ord[i]] = i;
   std::vector<Res> res;
   int last = partial >= 0 ? partial : n_reads;
   for (int i = 0; i < last; i++) {
     size_t before = W.calls.size();
     int nodes = (int)G.node.size(), edges = G.n_edges();
-    int rc = alignment(P, G, &S, i, seq + off[i], (int)(off[i + 1] - off[i]), res, &W.calls);
+    const int rid = ord[i];
+    int rc = alignment(P, G, &S, rid, seq + off[rid], (int)(off[rid + 1] - off[rid]), res,
+                       &W.calls);
     for (size_t c = before; c < W.calls.size(); c++) {
       W.call_read.push_back(i);
       W.call_nodes.push_back(nodes);
@@ -1281,9 +1377,9 @@ static int run_window(const Params& P, int n_reads, const char* seq, const int64
     o->put(G.cons);
     o->put('\n');
   } else if (P.result == 1) {
-    output_msa(G, P, n_reads, name, comment, *o);
+    output_msa(G, P, n_reads, name, comment, *o, &rid_to_ord);
   } else if (P.result == 3) {
-    output_msa_cons(G, P, n_reads, name, comment, *o);
+    output_msa_cons(G, P, n_reads, name, comment, *o, &rid_to_ord);
   } else if (P.result == 2) {
     std::vector<int> len(n_reads);
     for (int i = 0; i < n_reads; i++) len[i] = (int)(off[i + 1] - off[i]);
@@ -1523,6 +1619,7 @@ struct orc_dag {
   orc::Seeder S;
   bool seeded = false;
   std::vector<int> read_len;  // reads added so far (add_path), in order
+  std::vector<int> ord, rid_to_ord;  // the read set's order (-p: the guide tree), and inverse
 };
 
 static void put_res(const std::vector<orc::Res>& res, int32_t* out, int64_t cap, int64_t* n) {
@@ -1545,7 +1642,23 @@ extern "C" orc_dag* orc_dag_new(const orc_params* p, int32_t n_reads, const char
     g->S.init(g->P, n_reads, seq, seq_off);
     g->seeded = true;
   }
+  if (n_reads > 0 && seq && seq_off) {
+    if (g->P.progressive) {
+      g->ord = orc::guide_tree(g->P, n_reads, seq, seq_off);
+    } else {
+      g->ord.resize(n_reads);
+      for (int i = 0; i < n_reads; i++) g->ord[i] = i;
+    }
+    g->rid_to_ord.resize(n_reads);
+    for (int i = 0; i < n_reads; i++) g->rid_to_ord[g->ord[i]] = i;
+  }
   return g;
+}
+
+extern "C" int orc_dag_order(orc_dag* g, int32_t* ord, int32_t cap) {
+  if (!g) return ORC_E_ARG;
+  for (int i = 0; i < (int)g->ord.size() && i < cap; i++) ord[i] = g->ord[i];
+  return (int)g->ord.size();
 }
 
 extern "C" void orc_dag_free(orc_dag* g) { delete g; }
@@ -1607,10 +1720,11 @@ extern "C" int orc_dag_output(orc_dag* g, int32_t result, const char* name,
     o.put(">consensus sequence\n");
     o.put(g->G.cons);
     o.put('\n');
-  } else if (result == 1) {
-    orc::output_msa(g->G, g->P, n, nm, cm, o);
-  } else if (result == 3) {
-    orc::output_msa_cons(g->G, g->P, n, nm, cm, o);
+  } else if (result == 1 || result == 3) {
+    // rows through the read set's order when every read of it has been added
+    const std::vector<int>* map = (int)g->rid_to_ord.size() == n ? &g->rid_to_ord : nullptr;
+    if (result == 1) orc::output_msa(g->G, g->P, n, nm, cm, o, map);
+    else orc::output_msa_cons(g->G, g->P, n, nm, cm, o, map);
   } else {
     orc::output_gfa(g->G, n, g->read_len, nm, o);
   }

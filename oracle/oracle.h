@@ -33,7 +33,9 @@ typedef struct orc_params {
   int32_t seeding;                            /* -S */
   int32_t k, mm_w;                            /* -k -w (defaults 19 10) */
   int32_t poa_w;                              /* -W; 0 = flag not given */
-  int32_t result;                             /* -r: 0 consensus, 1 row-column MSA, 2 GFA */
+  int32_t result;                             /* -r: 0 consensus, 1 row-column MSA, 2 GFA,
+                                                 3 the MSA with a consensus row */
+  int32_t progressive;                        /* -p: guide-tree read order */
 } orc_params;
 
 /* One poa() call: the DP geometry and its optimum. */
@@ -116,6 +118,9 @@ typedef struct orc_dag orc_dag;
 orc_dag* orc_dag_new(const orc_params* p, int32_t n_reads, const char* seq,
                          const int64_t* seq_off);
 void orc_dag_free(orc_dag* g);
+/* The read set's order (-p: minimizer_t::get_guide_tree, minimizer.cpp:195-386; else input
+ * order): ord[i] is the read added i-th. Returns the number of reads. */
+int orc_dag_order(orc_dag* g, int32_t* ord, int32_t cap);
 int orc_dag_poa(orc_dag* g, int32_t beg_id, int32_t end_id, const char* q, int32_t qlen,
                   int32_t ab_band, int32_t* res, int64_t cap, int64_t* n_res, int32_t* score);
 int orc_dag_alignment(orc_dag* g, int32_t rid, const char* q, int32_t qlen, int32_t* res,

@@ -71,7 +71,7 @@ bool read_fastx(const char* path, std::vector<Read>& reads) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  orc_params p = {2, -4, -4, -2, 100, 40, 0, 0, 19, 10, 0, 0};
+  orc_params p = {2, -4, -4, -2, 100, 40, 0, 0, 19, 10, 0, 0, 0};
   int verbose = 0;
   const char* path = nullptr;
   for (int a = 1; a < argc; a++) {
@@ -108,6 +108,7 @@ int main(int argc, char** argv) {
     else if (key == "f" || key == "band_f") p.band_f = num();
     else if (key == "B" || key == "ab_band") p.ab_band = 1;
     else if (key == "S" || key == "seeding") p.seeding = 1;
+    else if (key == "p" || key == "progressive_poa") p.progressive = 1;
     else if (key == "W" || key == "poa_w") p.poa_w = num();
     else if (key == "k" || key == "k_mer") p.k = num();
     else if (key == "w" || key == "mm_w") p.mm_w = num();

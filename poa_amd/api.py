@@ -30,7 +30,8 @@ EXPORTS = [
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
     "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph", "mpoa_batch_set_forms",
     "mpoa_set_forms", "mpoa_align_reads_noalloc", "mpoa_graph_create", "mpoa_graph_destroy",
-    "mpoa_poa", "mpoa_seeder_create", "mpoa_seeder_destroy", "mpoa_alignment",
+    "mpoa_poa", "mpoa_seeder_create", "mpoa_seeder_destroy", "mpoa_seeder_order",
+    "mpoa_alignment",
     "mpoa_graph_add_path", "mpoa_graph_topsort", "mpoa_graph_output", "mpoa_graph_export",
     "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
@@ -54,12 +55,12 @@ class Params(ctypes.Structure):
     """minipoa flags; defaults are minipoa's (src/main.cpp:18-33)."""
     _fields_ = [(n, ctypes.c_int32) for n in (
         "match", "mismatch", "gap_open", "gap_ext", "band_b", "band_f", "ab_band", "seeding",
-        "k", "mm_w", "poa_w", "result")]
+        "k", "mm_w", "poa_w", "result", "progressive")]
 
     def __init__(self, match=2, mismatch=-4, gap_open=-4, gap_ext=-2, band_b=100, band_f=40,
-                 ab_band=0, seeding=0, k=19, mm_w=10, poa_w=0, result=0):
+                 ab_band=0, seeding=0, k=19, mm_w=10, poa_w=0, result=0, progressive=0):
         super().__init__(match, mismatch, gap_open, gap_ext, band_b, band_f, ab_band, seeding,
-                         k, mm_w, poa_w, result)
+                         k, mm_w, poa_w, result, progressive)
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -140,7 +141,9 @@ def lib() -> ctypes.CDLL:
         L.mpoa_alignment.argtypes = [VP, VP, I32, VP, I32, VP, I64, P(I64)]
         L.mpoa_graph_add_path.argtypes = [VP, I32, VP, I64]
         L.mpoa_graph_topsort.argtypes = [VP, I32]
-        L.mpoa_graph_output.argtypes = [VP, I32, VP, VP, VP, VP, VP, I64, P(I64)]
+        L.mpoa_graph_output.argtypes = [VP, I32, VP, VP, VP, VP, VP, VP, I64, P(I64)]
+        L.mpoa_seeder_order.argtypes = [VP, VP, I32]
+        L.mpoa_seeder_order.restype = I32
         L.mpoa_graph_export.argtypes = [VP, P(I32), P(I32)] + [VP] * 12
         _lib = L
     return _lib
@@ -360,6 +363,13 @@ class Seeder:
                                         ctypes.byref(h)))
         self.h = h
 
+    def order(self) -> np.ndarray:
+        """The order reads are added in: minipoa's guide tree with -p, else input order."""
+        n = lib().mpoa_seeder_order(self.h, None, 0)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().mpoa_seeder_order(self.h, _ptr(out), n)
+        return out[:n]
+
     def close(self) -> None:
         if self.h:
             lib().mpoa_seeder_destroy(self.h)
@@ -429,15 +439,16 @@ class Graph:
         _check(lib().mpoa_graph_topsort(self.h, op))
 
     def output(self, result: int, names: list[str] | None = None,
-               comments: list[str] | None = None) -> str:
+               comments: list[str] | None = None, rid_to_ord: np.ndarray | None = None) -> str:
         nb, no = _names(names)
         cb, co = _names(comments)
+        ro = None if rid_to_ord is None else np.ascontiguousarray(rid_to_ord, np.int32)
         n = ctypes.c_int64()
         _check(lib().mpoa_graph_output(self.h, result, _ptr(nb), _ptr(no), _ptr(cb), _ptr(co),
-                                       None, 0, ctypes.byref(n)))
+                                       _ptr(ro), None, 0, ctypes.byref(n)))
         buf = np.zeros(max(n.value, 1), np.uint8)
         _check(lib().mpoa_graph_output(self.h, result, _ptr(nb), _ptr(no), _ptr(cb), _ptr(co),
-                                       _ptr(buf), len(buf), ctypes.byref(n)))
+                                       _ptr(ro), _ptr(buf), len(buf), ctypes.byref(n)))
         return buf[:n.value].tobytes().decode()
 
     def export(self) -> dict:

@@ -5,7 +5,8 @@
 //                                                   their outputs, in argument order
 //
 // Options as minipoa: -M -X -O -E scores, -b -f band, -B adaptive band, -S seeding with -k -w
-// -W, -r 0|1|2 consensus / row-column MSA / GFA, -V 1 prints "score:<n>" per DP call to stderr
+// -W, -p progressive order, -r 0|1|2 consensus / row-column MSA / GFA (3: MSA with a consensus
+// row), -V 1 prints "score:<n>" per DP call to stderr
 // (src/align.cpp:1577), -t is accepted and ignored (one device). Exit status follows minipoa:
 // 0 after a band violation (align.cpp:1584-1588), 1 after a backtrack failure (:1730-1731).
 #include <zlib.h>
@@ -91,7 +92,7 @@ void usage() {
                "  -E INT gap extend (-2) -b INT band b (100)  -f INT band f (40)\n"
                "  -B adaptive band      -S seeding           -k INT k-mer (19)\n"
                "  -w INT window (10)    -W INT sub-window     -r INT 0 consensus, 1 MSA, 2 GFA, 3 MSA + consensus row\n"
-               "  -V INT verbosity      -t INT threads (ignored)\n");
+               "  -p progressive order  -V INT verbosity      -t INT threads (ignored)\n");
 }
 
 int exit_status(int rc) {
@@ -103,7 +104,7 @@ int exit_status(int rc) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  mpoa_params p = {2, -4, -4, -2, 100, 40, 0, 0, 19, 10, 0, 0};
+  mpoa_params p = {2, -4, -4, -2, 100, 40, 0, 0, 19, 10, 0, 0, 0};
   int verbose = 0;
   std::vector<const char*> files;
   for (int i = 1; i < argc; i++) {
@@ -130,6 +131,7 @@ int main(int argc, char** argv) {
       case 'f': p.band_f = value(); break;
       case 'B': p.ab_band = 1; break;
       case 'S': p.seeding = 1; break;
+      case 'p': p.progressive = 1; break;
       case 'k': p.k = value(); break;
       case 'w': p.mm_w = value(); break;
       case 'W': p.poa_w = value(); break;

@@ -29,6 +29,7 @@ using namespace mpoa::host;
 struct mpoa_seeder {
   mpoa::seed::Seeder sd;
   int n_reads = 0;
+  std::vector<int> ord;  // the order reads are added in (-p: the guide tree)
 };
 
 struct mpoa_graph {
@@ -341,11 +342,23 @@ int mpoa_seeder_create(const mpoa_params* params, int32_t n_reads, const char* s
   cfg.poa_w = params->poa_w;
   s->sd.init(cfg, n_reads, codes.data(), off.data());
   s->n_reads = n_reads;
+  if (params->progressive) {
+    s->ord = mpoa::seed::guide_order(cfg, n_reads, codes.data(), off.data());
+  } else {
+    s->ord.resize((size_t)n_reads);
+    for (int i = 0; i < n_reads; i++) s->ord[i] = i;
+  }
   *out = s;
   return MPOA_OK;
 }
 
 void mpoa_seeder_destroy(mpoa_seeder* s) { delete s; }
+
+int32_t mpoa_seeder_order(mpoa_seeder* s, int32_t* ord, int32_t cap) {
+  if (!s) return MPOA_E_ARG;
+  for (int i = 0; i < s->n_reads && i < cap && ord; i++) ord[i] = s->ord[i];
+  return s->n_reads;
+}
 
 int mpoa_alignment(mpoa_graph* g, mpoa_seeder* mm, int32_t rid, const char* qseq, int32_t qlen,
                    mpoa_res* res, int64_t cap, int64_t* n_res) {
@@ -467,8 +480,8 @@ int mpoa_graph_topsort(mpoa_graph* g, int32_t op) {
 }
 
 int mpoa_graph_output(mpoa_graph* g, int32_t result, const char* name, const int64_t* name_off,
-                      const char* comment, const int64_t* comment_off, char* out, int64_t out_cap,
-                      int64_t* out_len) {
+                      const char* comment, const int64_t* comment_off, const int32_t* rid_to_ord,
+                      char* out, int64_t out_cap, int64_t* out_len) {
   if (!g || result < 0 || result > 3) return MPOA_E_ARG;
   if (result != 0 && g->params.result == 0) {
     g_err = "MSA and GFA output need a graph created with result 1, 2 or 3 (read paths kept)";
@@ -510,8 +523,23 @@ int mpoa_graph_output(mpoa_graph* g, int32_t result, const char* name, const int
   WinResult r{};
   HIPCHK(hipMemcpy(&r, g->d_res.p, sizeof r, hipMemcpyDeviceToHost));
   std::string text;
+  if (rid_to_ord) {
+    std::vector<char> seen((size_t)n, 0);
+    for (int i = 0; i < n; i++) {
+      if (rid_to_ord[i] < 0 || rid_to_ord[i] >= n || seen[rid_to_ord[i]]) {
+        g_err = "rid_to_ord is not a permutation of the reads added";
+        return MPOA_E_ARG;
+      }
+      seen[rid_to_ord[i]] = 1;
+    }
+    if (result == 2) {
+      g_err = "a read order with GFA output: minipoa's GFA writer indexes its read paths past "
+              "their ends then (graph.cpp:369,423)";
+      return MPOA_E_ARG;
+    }
+  }
   if ((rc = window_text(result, r, g->d_text.p, g->d_gfa.p, n, name, name_off, comment,
-                        comment_off, &text)) != MPOA_OK)
+                        comment_off, rid_to_ord, &text)) != MPOA_OK)
     return rc;
   if (out_len) *out_len = (int64_t)text.size();
   if (!out) return MPOA_OK;

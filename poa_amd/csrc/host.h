@@ -138,9 +138,10 @@ inline Estimate worst_case(const WinInfo& w) {
 // minipoa's stdout for a finished window (result 0: ">consensus sequence" + bases; 1: the
 // row-column MSA with its headers; 2: GFA with read names on the path lines) from the device
 // output region d_out (and, for GFA, the path-line offsets d_gfa) of the result record r.
+// row_of (optional): the MSA row read i prints, when reads were added in another order (-p).
 int window_text(int result, const mpoa::WinResult& r, const uint8_t* d_out, const int64_t* d_gfa,
                 int n_reads, const char* name, const int64_t* name_off, const char* comment,
-                const int64_t* comment_off, std::string* out);
+                const int64_t* comment_off, const int* row_of, std::string* out);
 
 }  // namespace host
 }  // namespace mpoa

@@ -1,6 +1,6 @@
 // minipoa's main loop (src/main.cpp:84-126) written against the C++ facade
 // include/minipoa_amd.hpp: the program a minipoa user keeps, with the alignment path on the
-// GPU. Reads one plain FASTA file; flags -M -X -O -E -b -f -B -S -k -w -r as minipoa's
+// GPU. Reads one plain FASTA file; flags -M -X -O -E -b -f -B -S -k -w -p -r as minipoa's
 // (src/main.cpp:14-35). Prints what minipoa prints.
 #include <cstdio>
 #include <cstdlib>
@@ -49,6 +49,7 @@ int main(int argc, char** argv) {
     else if (a == "-f") para->f = val();
     else if (a == "-B") para->ab_band = true;
     else if (a == "-S") para->enable_seeding = true;
+    else if (a == "-p") para->progressive_poa = true;
     else if (a == "-k") para->k = val();
     else if (a == "-w") para->mm_w = val();
     else if (a == "-r") para->result = val();
@@ -63,6 +64,7 @@ int main(int argc, char** argv) {
     graph* DAG = new graph();
     DAG->init(para);
     minimizer_t* mm = new minimizer_t(para, seqs);
+    mm->get_guide_tree(para);
     for (int i = 0; i < (int)seqs.size(); i++) {
       const int rid = mm->ord[i];
       std::vector<res_t> res =

@@ -76,6 +76,11 @@ struct Bin {
 struct mpoa_batch {
   mpoa_ctx* ctx = nullptr;
   mpoa_params params{};
+  // -p: the reads permuted into each window's guide-tree order (host copy, what was uploaded)
+  // and, per read in input order, the MSA row it prints (minipoa's rid_to_ord)
+  std::string perm_seq;
+  std::vector<int64_t> perm_off;
+  std::vector<int> row_of;
   DpScores sc{};
   int n_windows = 0;
   std::vector<WinInfo> info;
@@ -799,7 +804,7 @@ namespace host {
 
 int window_text(int result, const WinResult& r, const uint8_t* d_out, const int64_t* d_gfa,
                 int n_reads, const char* name, const int64_t* name_off, const char* comment,
-                const int64_t* comment_off, std::string* out) {
+                const int64_t* comment_off, const int* row_of, std::string* out) {
   std::string& text = *out;
   text.clear();
   if (result == 0) {
@@ -851,7 +856,8 @@ int window_text(int result, const WinResult& r, const uint8_t* d_out, const int6
       if (comment && comment_off) cm.assign(comment + comment_off[i], comment + comment_off[i + 1]);
       if (nm.empty()) text += ">" + std::to_string(i + 1) + "\n";
       else text += ">" + nm + " " + cm + "\n";
-      text.append(rows.data() + (int64_t)i * L, (size_t)L);
+      // read i prints the row of the read added at position row_of[i] (graph.cpp:249)
+      text.append(rows.data() + (int64_t)(row_of ? row_of[i] : i) * L, (size_t)L);
       text.push_back('\n');
     }
     if (result == 3) {
@@ -947,11 +953,46 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
     g_err = "result must be 0 (consensus), 1 (MSA), 2 (GFA) or 3 (MSA with a consensus row)";
     return MPOA_E_ARG;
   }
+  if (params->progressive && params->result == 2) {
+    g_err = "progressive order (-p) with GFA output: minipoa's GFA writer indexes its read "
+            "paths past their ends then (graph.cpp:369,423)";
+    return MPOA_E_ARG;
+  }
   HIPCHK(hipSetDevice(ctx->device));
   mpoa_batch* b = new mpoa_batch();
   b->ctx = ctx;
   b->forms = ctx->forms;
   b->params = *params;
+  if (params->progressive && n_windows > 0) {
+    // each window's reads in minipoa's guide-tree order (minimizer.cpp:195-386); the device
+    // sees them in that order, the outputs map rows back
+    const int r0 = win_off[0], r1 = win_off[n_windows];
+    b->perm_off.assign(1, 0);
+    b->row_of.assign((size_t)(r1 - r0), 0);
+    seed::Config cfg;
+    cfg.k = params->k;
+    cfg.w = params->mm_w;
+    for (int w = 0; w < n_windows; w++) {
+      const int a = win_off[w], n = win_off[w + 1] - win_off[w];
+      std::vector<int64_t> off((size_t)n + 1, 0);
+      for (int i = 0; i < n; i++) off[i + 1] = off[i] + (seq_off[a + i + 1] - seq_off[a + i]);
+      std::vector<uint8_t> codes((size_t)off[n] + 1);
+      for (int64_t j = 0; j < off[n]; j++)
+        codes[j] = code_of_char((unsigned char)seq[seq_off[a] + j]);
+      const std::vector<int> ord = seed::guide_order(cfg, n, codes.data(), off.data());
+      for (int i = 0; i < n; i++) {
+        const int rid = a + ord[i];
+        b->perm_seq.append(seq + seq_off[rid], (size_t)(seq_off[rid + 1] - seq_off[rid]));
+        b->perm_off.push_back((int64_t)b->perm_seq.size());
+        b->row_of[(size_t)(a - r0 + ord[i])] = i;
+      }
+    }
+    seq = b->perm_seq.data();
+    std::vector<int64_t> shifted((size_t)r1 + 1, 0);
+    for (int r = r0; r <= r1; r++) shifted[r] = b->perm_off[r - r0];
+    b->perm_off.swap(shifted);
+    seq_off = b->perm_off.data();
+  }
   b->sc = scores_of(*params);
   b->n_windows = n_windows;
   b->info.resize(n_windows);
@@ -1187,9 +1228,10 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
       text.assign((const char*)src, (size_t)std::max(0, r.out_len));
     } else {
       const int L = std::max(0, r.out_len);
+      const int* row_of = b->row_of.empty() ? nullptr : b->row_of.data() + b->desc[w].read0;
       for (int i = 0; i < b->info[w].n_reads; i++) {
         text += ">" + std::to_string(i + 1) + "\n";
-        text.append((const char*)src + (int64_t)i * L, (size_t)L);
+        text.append((const char*)src + (int64_t)(row_of ? row_of[i] : i) * L, (size_t)L);
         text.push_back('\n');
       }
       if (b->params.result == 3) {
@@ -1440,7 +1482,9 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   }
   std::vector<mpoa_call_stat> calls;
   if (seeding) {
-    rc = run_seeded(b, seq, so, &calls);
+    // with -p the batch holds the reads in guide-tree order; the seeding follows that order
+    const bool perm = !b->row_of.empty();
+    rc = run_seeded(b, perm ? b->perm_seq.data() : seq, perm ? b->perm_off.data() : so, &calls);
   } else {
     b->split = false;  // one window: the wide-tile launch
     assign_bins(b);
@@ -1466,7 +1510,8 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   const WinResult r = b->res.empty() ? WinResult{} : b->res[0];
   std::string text;
   rc = mpoa::host::window_text(params->result, r, b->d_out.p, b->d_gfa.p, n_reads, name,
-                               name_off, comment, comment_off, &text);
+                               name_off, comment, comment_off,
+                               b->row_of.empty() ? nullptr : b->row_of.data(), &text);
   if (rc != MPOA_OK) {
     mpoa_batch_free(b);
     return rc;

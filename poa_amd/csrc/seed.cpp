@@ -349,6 +349,89 @@ std::vector<Pair> global_chain(const std::vector<Pair>& lc, std::vector<Pair>& a
 
 }  // namespace
 
+std::vector<int> guide_order(const Config& c, int n_reads, const uint8_t* codes,
+                             const int64_t* off) {
+  std::vector<int> ord(n_reads);
+  for (int i = 0; i < n_reads; i++) ord[i] = i;
+  if (n_reads < 2) return ord;
+  std::vector<Pair> mins;
+  std::vector<int> len(n_reads);
+  int maxl = 0;
+  for (int i = 0; i < n_reads; i++) {
+    len[i] = (int)(off[i + 1] - off[i]);
+    maxl = std::max(maxl, len[i]);
+    sketch(codes + off[i], len[i], c.w, c.k, (uint32_t)i, mins);
+  }
+  if (mins.empty()) return ord;
+  // minimizers grouped by hash: each read's total, and for every pair of reads the smaller of
+  // their counts in each group, summed (minimizer.cpp:211-259)
+  sort_x(mins.data(), mins.data() + mins.size());
+  const size_t n = (size_t)n_reads;
+  std::vector<int64_t> total(n, 0), shared(n * n, 0);
+  std::vector<int> cnt(n, 0), present;
+  for (size_t g0 = 0; g0 < mins.size();) {
+    size_t g1 = g0;
+    while (g1 < mins.size() && mins[g1].x == mins[g0].x) g1++;
+    present.clear();
+    for (size_t t = g0; t < g1; t++) {
+      const int r = (int)(mins[t].y >> 32);
+      if (cnt[r]++ == 0) present.push_back(r);
+      total[r]++;
+    }
+    for (size_t a = 0; a < present.size(); a++)
+      for (size_t b = 0; b < present.size(); b++)
+        if (present[a] < present[b])
+          shared[(size_t)present[b] * n + present[a]] += std::min(cnt[present[a]], cnt[present[b]]);
+    for (int r : present) cnt[r] = 0;
+    g0 = g1;
+  }
+  // length-weighted Jaccard similarity of every ordered pair (minimizer.cpp:264-290): the
+  // weights make it asymmetric; the first pair of the largest value opens the order
+  std::vector<double> sim(n * n, 0.0);
+  double best = -1.0;
+  int bi = -1, bj = -1;
+  for (size_t i = 0; i < n; i++) {
+    for (size_t j = 0; j < n; j++) {
+      if (i == j) continue;
+      const int64_t sh = i > j ? shared[i * n + j] : shared[j * n + i];
+      const int64_t tot = total[i] + total[j] - sh;
+      double jac = 0.0;
+      if (tot > 0) {
+        jac = (0.0 + (double)sh) / (double)tot;
+        jac = jac * ((0.51 * len[i] / maxl) + (0.49 * len[j] / maxl));
+      }
+      sim[i * n + j] = jac;
+      if (jac > best) {
+        best = jac;
+        bi = (int)i;
+        bj = (int)j;
+      }
+    }
+  }
+  // then, one read at a time, the one whose similarities from the reads already ordered sum
+  // highest, summed in their order (minimizer.cpp:295-330)
+  ord[0] = bi;
+  ord[1] = bj;
+  std::vector<char> placed(n, 0);
+  placed[bi] = placed[bj] = 1;
+  for (int at = 2; at < n_reads; at++) {
+    double top = -1.0;
+    int pick = n_reads;
+    for (int r = 0; r < n_reads; r++) {
+      if (placed[r]) continue;
+      double sum = 0.0;
+      for (int t = 0; t < at; t++) sum += sim[(size_t)ord[t] * n + r];
+      if (sum > top) {
+        top = sum;
+        pick = r;
+      }
+    }
+    ord[at] = pick;
+    placed[pick] = 1;
+  }
+  return ord;
+}
+
 void Seeder::init(const Config& c, int n_reads, const uint8_t* codes, const int64_t* off) {
   cfg_ = c;
   n_reads_ = n_reads;

@@ -22,6 +22,14 @@ struct Config {
   int poa_w = 0;        // -W, 0 when not given
 };
 
+// Progressive order (-p, minipoa's guide tree, minimizer.cpp:195-386): read indices in the
+// order they are added to the graph. Reads are sketched with the seeding's k and w; pairs are
+// compared by their shared minimizers (length-weighted Jaccard similarity); the most similar
+// pair opens the order, then each next read is the one most similar in sum to those already
+// in it. Fewer than two reads, or none with a minimizer: input order.
+std::vector<int> guide_order(const Config& c, int n_reads, const uint8_t* codes,
+                             const int64_t* off);
+
 // One window's reads, sketched once; the consensus is sketched per call.
 class Seeder {
  public:

@@ -41,6 +41,7 @@ MTDNA_RUNS = [
     ("mtdna", ["-r", "1"]),
     ("mtdna", ["-r", "2"]),
     ("mtdna", ["-B", "-r", "0"]),
+    ("mtdna", ["-p", "-r", "1"]),
     # README.md's MSA command: upstream dies on a sub-window that ends before it begins
     ("mtdna", ["-S", "-t", "12", "-r", "1"]),
 ]

@@ -92,6 +92,18 @@ def cases():
         reads = [(f"e{i + 1}", "x y" if i == 0 else "", s) for i, s in enumerate(rs)]
         for args in (["-r", "0"], ["-r", "1"], ["-r", "2"]):
             c.append((f"edge_{name}", reads, args))
+    # progressive order (-p, minimizer.cpp:195-386); its GFA output reads past minipoa's own
+    # path arrays when read lengths differ (graph.cpp:369,423), so -r 2 is left out
+    for args in (["-p", "-r", "0"], ["-p", "-r", "1"], ["-p", "-B", "-r", "1"]):
+        c.append(("mtdna40", mt, args))
+    for args in (["-p", "-r", "0"], ["-p", "-r", "1"]):
+        c.append(("cfg2_small", c2, args))
+    c.append(("cfg2_seeded", c2s, ["-p", "-S", "-r", "1"]))
+    for w in range(2):
+        c.append((f"cfg3_w{w}", synth_reads(b.window(w), f"w{w}_"), ["-p", "-r", "0"]))
+    for name in ("two_reads", "short", "identical"):
+        reads = [(f"e{i + 1}", "x y" if i == 0 else "", s) for i, s in enumerate(edges[name])]
+        c.append((f"edge_{name}", reads, ["-p", "-r", "1"]))
     return c
 
 

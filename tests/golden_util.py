@@ -15,7 +15,7 @@ _FLAGS = {
     "-M": ("match", True), "-X": ("mismatch", True), "-O": ("gap_open", True),
     "-E": ("gap_ext", True), "-b": ("band_b", True), "-f": ("band_f", True),
     "-B": ("ab_band", False), "-S": ("seeding", False), "-k": ("k", True), "-w": ("mm_w", True),
-    "-W": ("poa_w", True), "-r": ("result", True),
+    "-W": ("poa_w", True), "-r": ("result", True), "-p": ("progressive", False),
 }
 
 

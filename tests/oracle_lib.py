@@ -17,7 +17,7 @@ ORACLE_CLI = os.path.join(ROOT, "oracle", "oracle_cli")
 class OParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "match", "mismatch", "gap_open", "gap_ext", "band_b", "band_f", "ab_band", "seeding",
-        "k", "mm_w", "poa_w", "result")]
+        "k", "mm_w", "poa_w", "result", "progressive")]
 
 
 class OCallStat(ctypes.Structure):
@@ -44,7 +44,7 @@ def oparams(p) -> OParams:
     """From poa_amd.api.Params (or a dict) to the oracle's struct."""
     d = p.as_dict() if hasattr(p, "as_dict") else dict(p)
     base = dict(match=2, mismatch=-4, gap_open=-4, gap_ext=-2, band_b=100, band_f=40,
-                ab_band=0, seeding=0, k=19, mm_w=10, poa_w=0, result=0)
+                ab_band=0, seeding=0, k=19, mm_w=10, poa_w=0, result=0, progressive=0)
     base.update(d)
     return OParams(**base)
 
@@ -189,6 +189,13 @@ class Dag:
             lib().orc_dag_free(self.h)
             self.h = None
 
+    def order(self) -> np.ndarray:
+        """The order reads are added in: minipoa's guide tree with -p, else input order."""
+        n = lib().orc_dag_order(ctypes.c_void_p(self.h), None, 0)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_dag_order(ctypes.c_void_p(self.h), _p(out), n)
+        return out[:n]
+
     def _res(self, fn, *args) -> np.ndarray:
         n = ctypes.c_int64()
         rc = fn(ctypes.c_void_p(self.h), *args, None, ctypes.c_int64(0), ctypes.byref(n))
@@ -268,15 +275,22 @@ class Dag:
 
 
 def run_loop(dag, n_reads: int, seq: np.ndarray, off: np.ndarray, result: int,
-             names=None, comments=None, seeder=None):
-    """minipoa's main loop (src/main.cpp:110-126) driven through a call-level object (the
-    oracle's Dag or the product's api.Graph): per read alignment -> add_path -> topsort
-    (op 1 after the last read), then the output. Returns (text, [res of every read])."""
+             names=None, comments=None, seeder=None, order=None):
+    """minipoa's main loop (src/main.cpp:100-126) driven through a call-level object (the
+    oracle's Dag or the product's api.Graph): per read, in `order` (the guide tree with -p),
+    alignment -> add_path -> topsort (op 1 after the last read), then the output, whose MSA
+    rows map back through the order. Returns (text, [res of every read])."""
+    ord_ = list(range(n_reads)) if order is None else [int(x) for x in order]
+    rid_to_ord = np.zeros(n_reads, np.int32)
+    for i, r in enumerate(ord_):
+        rid_to_ord[r] = i
     all_res = []
-    for i in range(n_reads):
-        q = bytes(seq[off[i]:off[i + 1]])
-        res = dag.alignment(i, q) if seeder is None else dag.alignment(i, q, seeder)
+    for i, rid in enumerate(ord_):
+        q = bytes(seq[off[rid]:off[rid + 1]])
+        res = dag.alignment(rid, q) if seeder is None else dag.alignment(rid, q, seeder)
         all_res.append(res)
         dag.add_path(i, res)
         dag.topsort(1 if i + 1 == n_reads else 0)
-    return dag.output(result, names, comments), all_res
+    if seeder is None:
+        return dag.output(result, names, comments), all_res
+    return dag.output(result, names, comments, rid_to_ord), all_res

@@ -30,18 +30,24 @@ def test_call_level_loop(ctx, case):
     n = len(off) - 1
     dag = ol.Dag(p, seq, off)
     g = api.Graph(ctx, params)
-    seeder = api.Seeder(params, seq, off) if params.seeding else None
+    seeder = api.Seeder(params, seq, off)
+    order = seeder.order()
+    np.testing.assert_array_equal(order, dag.order())
+    rid_to_ord = np.zeros(n, np.int32)
     for i in range(n):
-        q = bytes(seq[off[i]:off[i + 1]])
-        want = dag.alignment(i, q)
-        got = g.alignment(i, q, seeder)
-        np.testing.assert_array_equal(res_rows(got), want, err_msg=f"read {i}")
+        rid = int(order[i])
+        rid_to_ord[rid] = i
+        q = bytes(seq[off[rid]:off[rid + 1]])
+        want = dag.alignment(rid, q)
+        got = g.alignment(rid, q, seeder)
+        np.testing.assert_array_equal(res_rows(got), want, err_msg=f"read {rid}")
         dag.add_path(i, want)
         g.add_path(i, got)
         op = 1 if i + 1 == n else 0
         dag.topsort(op)
         g.topsort(op)
-    assert g.output(params.result, names, comments) == case["stdout"]
+    ro = rid_to_ord if params.progressive else None
+    assert g.output(params.result, names, comments, ro) == case["stdout"]
     g.close()
 
 

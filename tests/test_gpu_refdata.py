@@ -146,3 +146,19 @@ def test_device_logical_cells(ctx, case):
         b.execute()
         assert b.stats().logical_cells == sum(w.logical_cells for w in want)
         b.free()
+
+
+def test_device_progressive_batch(ctx):
+    """-p in the batch: every window's reads in its own guide-tree order, MSA rows back in
+    input order, equal to the oracle window by window; -p with GFA output is refused."""
+    from poa_amd import synth
+    b = synth.config3(seed=0x9f, n_windows=20)
+    for result in (1, 3, 0):
+        params = api.Params(result=result, progressive=1)
+        bat = api.Batch(ctx, params, b.win_off, b.seq, b.seq_off)
+        bat.execute()
+        assert bat.results().texts == oracle_lib.batch(params, b.win_off, b.seq, b.seq_off)
+        bat.free()
+    with pytest.raises(api.MpoaError) as e:
+        api.Batch(ctx, api.Params(result=2, progressive=1), b.win_off, b.seq, b.seq_off)
+    assert e.value.rc == -1

@@ -18,7 +18,8 @@ def test_call_level_loop_matches_fixture(case):
     p = gu.parse_args(case["args"])
     seq, off, names, comments = gu.parse_fasta(case["fasta"])
     dag = ol.Dag(p, seq, off)
-    text, _ = ol.run_loop(dag, len(off) - 1, seq, off, ol.oparams(p).result, names, comments)
+    text, _ = ol.run_loop(dag, len(off) - 1, seq, off, ol.oparams(p).result, names, comments,
+                          order=dag.order())
     assert text == case["stdout"]
 
 

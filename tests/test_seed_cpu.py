@@ -62,3 +62,25 @@ def test_product_seeding_equals_oracle(case):
             assert np.array_equal(a, b), (rid, cons_src)
             total += len(a)
     assert total > 0
+
+
+def test_guide_tree_order_matches_oracle():
+    """The product's progressive order (-p, minimizer.cpp:195-386; host code, no GPU) equals the
+    oracle's on every -p fixture and on seeded read sets of uneven length."""
+    import golden_util as gu
+    import oracle_lib as ol
+    from poa_amd import api, synth
+    sets = []
+    for c in gu.load_cases():
+        if "-p" in c["args"]:
+            seq, off, _, _ = gu.parse_fasta(c["fasta"])
+            sets.append((gu.parse_args(c["args"]), seq, off))
+    b = synth.config3(seed=0x9e, n_windows=12)
+    for w in range(b.n_windows):
+        r = b.window(w)
+        sets.append(({"progressive": 1}, r.seq, r.off))
+    for p, seq, off in sets:
+        want = ol.Dag(p, seq, off).order()
+        got = api.Seeder(api.Params(**p), seq, off).order()
+        assert list(got) == list(want)
+        assert sorted(got) == list(range(len(off) - 1))
