@@ -20,7 +20,9 @@
 // first graph is created (default 0).
 #pragma once
 #include <cstdint>
+#include <fstream>
 #include <iostream>
+#include <iterator>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -69,7 +71,7 @@ struct para_t {
   bool enable_seeding = false;
   int bw = 1000, poa_w = 0;
   bool progressive_poa = false;
-  int result = 0;  // 0 consensus, 1 row-column MSA, 2 GFA
+  int result = 0;  // 0 consensus, 1 row-column MSA, 2 GFA, 3 MSA with a consensus row
   int verbose = 0;
 
   mpoa_params c_params() const {
@@ -198,6 +200,25 @@ struct graph {
   // graph::topsort(para, op) (src/graph.cpp:21-128)
   void topsort(const para_t* /*para*/, int op) { check(mpoa_graph_topsort(h, op)); }
 
+  // read_gfa's work on a GFA text (src/handle_input.cpp:444-495): segments and '+' paths into
+  // this new graph; the paths become the first reads, returned as seq_t with their names and
+  // no bases.
+  std::vector<seq_t> read_gfa_text(const std::string& text) {
+    std::string names(text.size() + 1, '\0');  // a name and its newline fit in its P line
+    int64_t len = 0;
+    int32_t n = 0;
+    check(mpoa_graph_read_gfa(h, text.data(), (int64_t)text.size(), &names[0],
+                              (int64_t)names.size(), &len, &n));
+    std::vector<seq_t> seqs((size_t)n);
+    size_t at = 0;
+    for (int i = 0; i < n; i++) {
+      const size_t nl = names.find('\n', at);
+      seqs[(size_t)i].name = names.substr(at, nl - at);
+      at = nl + 1;
+    }
+    return seqs;
+  }
+
   // The outputs as text, and printed to std::cout as minipoa prints them (graph.cpp:237-433).
   std::string text(int result, const std::vector<seq_t>* seqs = nullptr,
                    const std::vector<int>* rid_to_ord = nullptr) {
@@ -230,6 +251,16 @@ struct graph {
     std::cout << text(2, &seqs, &rid_to_ord);
   }
 };
+
+// read_gfa (src/handle_input.cpp:444-495, minipoa's -i): the plain-text GFA at `path` into DAG,
+// fresh from init; returns the paths as the read set's first seq_t entries. A file that cannot
+// be opened gives no entries, as in minipoa (:452-455).
+inline std::vector<seq_t> read_gfa(para_t* /*para*/, graph* DAG, const char* path) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return {};
+  const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  return DAG->read_gfa_text(text);
+}
 
 // poa() (include/align.h:15, src/align.cpp:1353-1737); qid and mpool are not used.
 inline std::vector<res_t> poa(const para_t* /*para*/, const graph* DAG, int beg_id, int end_id,

@@ -44,7 +44,8 @@ enum {
   MPOA_E_BAND = -2,         /* backtrack left the band (minipoa: exit(0), align.cpp:1584) */
   MPOA_E_BACKTRACK = -3,    /* no move found (minipoa: "backtrack error", exit(1), :1730) */
   MPOA_E_OUT_CAP = -4,      /* caller buffer too small; the size needed is reported */
-  MPOA_E_EMPTY_WINDOW = -5, /* -S sub-window ends before it begins (minipoa: SIGSEGV) */
+  MPOA_E_EMPTY_WINDOW = -5, /* -S sub-window ends before it begins, in ranks (minipoa:
+                               SIGSEGV) or in the read (minipoa: std::length_error abort) */
   MPOA_E_OOM = -6,          /* device allocation failed (minipoa: throw, mem_alloc_utils.cpp:27) */
   MPOA_E_DEVICE = -7,       /* HIP runtime error, or no GPU */
   MPOA_E_OVERFLOW = -8,     /* a window outgrew its pre-sized arena: internal to the other
@@ -128,7 +129,7 @@ void mpoa_graph_destroy(mpoa_graph* g);
  * column's match to beg_id), *n_res its length (also when MPOA_E_OUT_CAP is returned), *score
  * H[n-1][m] (minipoa's "score:" under -V 1). The graph must have been sorted (op 0) since its
  * last add_path. Errors: MPOA_E_BAND and MPOA_E_BACKTRACK where minipoa exits (align.cpp:1584,
- * :1730), MPOA_E_EMPTY_WINDOW for rank(end_id) < rank(beg_id). */
+ * :1730), MPOA_E_EMPTY_WINDOW for rank(end_id) < rank(beg_id) or qlen < 0. */
 int mpoa_poa(mpoa_graph* g, int32_t beg_id, int32_t end_id, const char* qseq, int32_t qlen,
              int32_t ab_band, mpoa_res* res, int64_t cap, int64_t* n_res, int32_t* score);
 
@@ -156,6 +157,17 @@ int mpoa_alignment(mpoa_graph* g, mpoa_seeder* mm, int32_t rid, const char* qseq
  * order 0, 1, 2, ... (the MSA and GFA rows follow it). Entries are checked on the host (node
  * ids below the node count, bases 0..4): MPOA_E_ARG otherwise. Marks the graph unsorted. */
 int mpoa_graph_add_path(mpoa_graph* g, int32_t seq_id, const mpoa_res* res, int64_t n_res);
+
+/* read_gfa (src/handle_input.cpp:444-495, minipoa's -i, called at src/main.cpp:86) into a
+ * graph that holds only the source and the sink: S lines add their bases as nodes (in file
+ * order), P lines add their paths as reads 0, 1, ... (one read per line, no bases of its own),
+ * other lines are skipped; then the op 0 sort. Only forward ('+') paths are read (minipoa's
+ * own GFA output has no others): a '-' segment, an unknown or duplicated segment, or a segment
+ * without a sequence is MPOA_E_ARG, where minipoa exits. names receives the path names, one per
+ * line; *names_len their size (also with MPOA_E_OUT_CAP), *n_paths their number. Reads added
+ * afterwards continue the numbering; MSA rows of the paths print under their names. */
+int mpoa_graph_read_gfa(mpoa_graph* g, const char* text, int64_t len, char* names,
+                        int64_t names_cap, int64_t* names_len, int32_t* n_paths);
 
 /* graph::topsort(para, op) (src/graph.cpp:21-128, called at src/main.cpp:120): op 0, the
  * LIFO topological order with hlen/tlen (and, with -S, build_consensus); op 1, the order of

@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -298,10 +299,14 @@ static int poa(const Params& P, const Graph& G, int beg_id, int end_id, const ui
   const std::vector<int>& rank = G.rank;
   const int beg_i = node[beg_id].rank, end_i = node[end_id].rank;
   const int n = end_i - beg_i + 1, m = qlen + 1;
+  // a sub-window that ends before it begins in ranks: minipoa writes to an empty vector
   if (n <= 0) return ORC_E_EMPTY_WINDOW;
 
-  // Query: sentinel N, the codes, N padding to col_size (align.cpp:1362-1373).
+  // Query: sentinel N, the codes, N padding to col_size (align.cpp:1362-1373). Overlapping -S
+  // anchors give qlen < 0: minipoa runs on down to qlen = -16 and its vector of negative size
+  // throws std::length_error below that.
   const int col_size = (m + 1 + Q - 1) / Q * Q;
+  if (col_size < 0) return ORC_E_EMPTY_WINDOW;
   std::vector<uint8_t> seq(col_size, CODE_N);
   for (int j = 0; j < qlen; j++) seq[1 + j] = code_of(q[j]);
 
@@ -1620,6 +1625,7 @@ struct orc_dag {
   bool seeded = false;
   std::vector<int> read_len;  // reads added so far (add_path), in order
   std::vector<int> ord, rid_to_ord;  // the read set's order (-p: the guide tree), and inverse
+  std::vector<int> scores;  // the score of every DP call completed on this graph (-V 1)
 };
 
 static void put_res(const std::vector<orc::Res>& res, int32_t* out, int64_t cap, int64_t* n) {
@@ -1666,9 +1672,10 @@ extern "C" void orc_dag_free(orc_dag* g) { delete g; }
 extern "C" int orc_dag_poa(orc_dag* g, int32_t beg_id, int32_t end_id, const char* q,
                              int32_t qlen, int32_t ab_band, int32_t* res, int64_t cap,
                              int64_t* n_res, int32_t* score) {
-  if (!g || qlen < 0 || beg_id < 0 || end_id < 0 || beg_id >= (int)g->G.node.size() ||
+  if (!g || beg_id < 0 || end_id < 0 || beg_id >= (int)g->G.node.size() ||
       end_id >= (int)g->G.node.size())
     return ORC_E_ARG;
+  if (qlen < 0) return ORC_E_EMPTY_WINDOW;
   std::vector<uint8_t> t(qlen);
   for (int j = 0; j < qlen; j++) t[j] = orc::code_of((uint8_t)q[j]);
   std::vector<orc::Res> out;
@@ -1676,6 +1683,7 @@ extern "C" int orc_dag_poa(orc_dag* g, int32_t beg_id, int32_t end_id, const cha
   const int rc = orc::poa(g->P, g->G, beg_id, end_id, t.data(), qlen, ab_band != 0, out, &co,
                           nullptr);
   if (score) *score = co.score;
+  if (rc == ORC_OK) g->scores.push_back(co.score);
   put_res(out, res, cap, n_res);
   return rc;
 }
@@ -1684,9 +1692,19 @@ extern "C" int orc_dag_alignment(orc_dag* g, int32_t rid, const char* q, int32_t
                                    int32_t* res, int64_t cap, int64_t* n_res) {
   if (!g || qlen < 0 || (g->P.seeding && !g->seeded)) return ORC_E_ARG;
   std::vector<orc::Res> out;
-  const int rc = orc::alignment(g->P, g->G, &g->S, rid, q, qlen, out, nullptr);
+  std::vector<orc::CallOut> calls;
+  const int rc = orc::alignment(g->P, g->G, &g->S, rid, q, qlen, out, &calls);
+  // the last call is the one that failed, if one did
+  for (size_t c = 0; c + (rc != ORC_OK ? 1 : 0) < calls.size(); c++)
+    g->scores.push_back(calls[c].score);
   put_res(out, res, cap, n_res);
   return rc;
+}
+
+extern "C" int orc_dag_scores(orc_dag* g, int32_t* out, int32_t cap) {
+  if (!g) return ORC_E_ARG;
+  for (int i = 0; i < (int)g->scores.size() && i < cap; i++) out[i] = g->scores[i];
+  return (int)g->scores.size();
 }
 
 extern "C" int orc_dag_add_path(orc_dag* g, int32_t seq_id, const int32_t* res,
@@ -1697,6 +1715,74 @@ extern "C" int orc_dag_add_path(orc_dag* g, int32_t seq_id, const int32_t* res,
     r[i] = {res[3 * i], (uint8_t)res[3 * i + 1], res[3 * i + 2]};
   g->G.add_path(seq_id, r, 1);
   if (seq_id == (int)g->read_len.size()) g->read_len.push_back((int)n_res - 1);
+  return ORC_OK;
+}
+
+// read_gfa (handle_input.cpp:444-495): S lines add their bases as nodes one by one
+// (gfa_parse_S, :78-127, graph::add_node at graph.cpp:7-12), P lines add their '+' paths edge
+// by edge with the read id of the path (gfa_parse_P, :128-261), then the op 0 sort (:482). The
+// inputs the library refuses ('-' segments, unknown or duplicated segments, segments without
+// bases, paths without segments, GFA output) are ORC_E_ARG here too.
+extern "C" int orc_dag_read_gfa(orc_dag* g, const char* text, int64_t len, char* names,
+                                  int64_t names_cap, int64_t* names_len, int32_t* n_paths) {
+  if (!g || len < 0 || g->G.node.size() != 2 || !g->read_len.empty() || g->P.result == 2)
+    return ORC_E_ARG;
+  std::map<std::string, std::pair<int, int>> seg;  // name -> (first node, last node)
+  std::string all;
+  int paths = 0;
+  int64_t at = 0;
+  while (at < len) {
+    int64_t e = at;
+    while (e < len && text[e] != '\n') e++;
+    std::string line(text + at, (size_t)(e - at));
+    at = e + 1;
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.size() < 3 || line[1] != '\t' || (line[0] != 'S' && line[0] != 'P')) continue;
+    const size_t t1 = line.find('\t', 2);
+    if (t1 == std::string::npos) return ORC_E_ARG;  // no second field
+    const std::string name = line.substr(2, t1 - 2);
+    const size_t t2 = line.find('\t', t1 + 1);
+    const std::string body = line.substr(t1 + 1, t2 == std::string::npos ? std::string::npos
+                                                                         : t2 - t1 - 1);
+    if (line[0] == 'S') {
+      if (body.empty() || seg.count(name)) return ORC_E_ARG;
+      const int first = (int)g->G.node.size();
+      for (char c : body) {
+        const int id = (int)g->G.node.size();
+        g->G.node.emplace_back(id, orc::code_of((uint8_t)c));
+      }
+      seg[name] = {first, (int)g->G.node.size() - 1};
+    } else if (line[0] == 'P') {
+      const int sid = (int)g->read_len.size();
+      int last = 0, pos = 0, visited = 0;
+      size_t from = 0;
+      for (size_t k = 0; k < body.size(); k++) {
+        if (body[k] == '-') return ORC_E_ARG;
+        if (body[k] != '+') continue;
+        auto it = seg.find(body.substr(from, k - from));
+        if (it == seg.end()) return ORC_E_ARG;
+        const int in = it->second.first, out = it->second.second;
+        g->G.add_adj(sid, last, in, pos++);
+        for (int v = in; v < out; v++) g->G.add_adj(sid, v, v + 1, pos++);
+        visited += out - in + 1;
+        last = out;
+        from = k + 2;
+      }
+      if (visited == 0) return ORC_E_ARG;
+      g->G.add_adj(sid, last, 1, pos);
+      g->G.is_topsorted = false;
+      g->read_len.push_back(visited);
+      all += name + "\n";
+      paths++;
+    }
+  }
+  g->G.topsort(g->P, 0);
+  if (n_paths) *n_paths = paths;
+  if (names_len) *names_len = (int64_t)all.size();
+  if (names) {
+    if ((int64_t)all.size() > names_cap) return ORC_E_OUT_CAP;
+    std::memcpy(names, all.data(), all.size());
+  }
   return ORC_OK;
 }
 

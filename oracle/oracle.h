@@ -126,7 +126,14 @@ int orc_dag_poa(orc_dag* g, int32_t beg_id, int32_t end_id, const char* q, int32
 int orc_dag_alignment(orc_dag* g, int32_t rid, const char* q, int32_t qlen, int32_t* res,
                         int64_t cap, int64_t* n_res);
 int orc_dag_add_path(orc_dag* g, int32_t seq_id, const int32_t* res, int64_t n_res);
+/* The scores of the DP calls completed on g so far, in order (minipoa's -V 1 "score:" lines);
+ * returns their number. */
+int orc_dag_scores(orc_dag* g, int32_t* out, int32_t cap);
 int orc_dag_topsort(orc_dag* g, int32_t op);
+/* read_gfa (src/handle_input.cpp:444-495) into a new graph: the GFA's '+' paths become reads
+ * 0, 1, ...; names receives their names, one per line. */
+int orc_dag_read_gfa(orc_dag* g, const char* text, int64_t len, char* names,
+                     int64_t names_cap, int64_t* names_len, int32_t* n_paths);
 int orc_dag_output(orc_dag* g, int32_t result, const char* name, const int64_t* name_off,
                      const char* comment, const int64_t* comment_off, char* out, int64_t out_cap,
                      int64_t* out_len);

@@ -18,10 +18,11 @@ import numpy as np
 from ._lib import PRODUCT_SO
 
 MPOA_OK = 0
+MPOA_E_OUT_CAP = -4
 FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE, FORM_CONS_ROUNDS = 1, 2, 4, 8, 16
 ERRORS = {
     -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
-    -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW",
+    -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW", -9: "E_SCRATCH",
 }
 
 EXPORTS = [
@@ -33,7 +34,7 @@ EXPORTS = [
     "mpoa_poa", "mpoa_seeder_create", "mpoa_seeder_destroy", "mpoa_seeder_order",
     "mpoa_alignment",
     "mpoa_graph_add_path", "mpoa_graph_topsort", "mpoa_graph_output", "mpoa_graph_export",
-    "mpoa_batch_set_predictions",
+    "mpoa_graph_read_gfa", "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
     "mpoa_mlp_train", "mpoa_mlp_error",
@@ -141,6 +142,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_alignment.argtypes = [VP, VP, I32, VP, I32, VP, I64, P(I64)]
         L.mpoa_graph_add_path.argtypes = [VP, I32, VP, I64]
         L.mpoa_graph_topsort.argtypes = [VP, I32]
+        L.mpoa_graph_read_gfa.argtypes = [VP, VP, I64, VP, I64, P(I64), P(I32)]
         L.mpoa_graph_output.argtypes = [VP, I32, VP, VP, VP, VP, VP, VP, I64, P(I64)]
         L.mpoa_seeder_order.argtypes = [VP, VP, I32]
         L.mpoa_seeder_order.restype = I32
@@ -405,11 +407,15 @@ class Graph:
             pass
 
     @staticmethod
-    def _res_call(fn, *args) -> np.ndarray:
+    def _res_call(fn, *args, guess: int = 64) -> np.ndarray:
+        """One call with room for `guess` entries; a second, sized one only on MPOA_E_OUT_CAP."""
         n = ctypes.c_int64()
-        _check(fn(*args, None, 0, ctypes.byref(n)))
-        out = np.zeros(max(n.value, 1), RES_DTYPE)
-        _check(fn(*args, _ptr(out), len(out), ctypes.byref(n)))
+        out = np.zeros(max(guess, 1), RES_DTYPE)
+        rc = fn(*args, _ptr(out), len(out), ctypes.byref(n))
+        if rc == MPOA_E_OUT_CAP:
+            out = np.zeros(max(n.value, 1), RES_DTYPE)
+            rc = fn(*args, _ptr(out), len(out), ctypes.byref(n))
+        _check(rc)
         return out[:n.value]
 
     def poa(self, beg_id: int, end_id: int, q: bytes, ab_band: bool):
@@ -426,7 +432,7 @@ class Graph:
         """alignment(): the res of read rid against the graph."""
         buf = ctypes.create_string_buffer(q, len(q) + 1)
         return self._res_call(lib().mpoa_alignment, self.h, seeder.h if seeder else None, rid,
-                              buf, len(q))
+                              buf, len(q), guess=2 * len(q) + 8)
 
     def add_path(self, seq_id: int, res: np.ndarray) -> None:
         """graph::add_path of a res given as RES_DTYPE records or as (n, 3) int32 rows."""
@@ -437,6 +443,17 @@ class Graph:
 
     def topsort(self, op: int) -> None:
         _check(lib().mpoa_graph_topsort(self.h, op))
+
+    def read_gfa(self, text: bytes) -> list[str]:
+        """read_gfa (minipoa -i): the GFA's segments and '+' paths into this new graph, the
+        paths as reads 0, 1, ...; returns the path names (a name and its newline take no more
+        room than its P line)."""
+        buf = ctypes.create_string_buffer(text, len(text) + 1)
+        out = ctypes.create_string_buffer(len(text) + 1)
+        n, k = ctypes.c_int64(), ctypes.c_int32()
+        _check(lib().mpoa_graph_read_gfa(self.h, buf, len(text), out, len(out), ctypes.byref(n),
+                                         ctypes.byref(k)))
+        return out.raw[:n.value].decode().split("\n")[:k.value]
 
     def output(self, result: int, names: list[str] | None = None,
                comments: list[str] | None = None, rid_to_ord: np.ndarray | None = None) -> str:

@@ -6,11 +6,14 @@
 //
 // Options as minipoa: -M -X -O -E scores, -b -f band, -B adaptive band, -S seeding with -k -w
 // -W, -p progressive order, -r 0|1|2 consensus / row-column MSA / GFA (3: MSA with a consensus
-// row), -V 1 prints "score:<n>" per DP call to stderr
-// (src/align.cpp:1577), -t is accepted and ignored (one device). Exit status follows minipoa:
-// 0 after a band violation (align.cpp:1584-1588), 1 after a backtrack failure (:1730-1731).
+// row), -i graph.gfa[.gz] aligns the reads to an existing graph (src/main.cpp:86-126; one read
+// set, call by call through the call-level API, -p ignored as minipoa ignores it), -V 1 prints
+// "score:<n>" per DP call to stderr (src/align.cpp:1577; not with -i), -t is accepted and
+// ignored (one device). Exit status follows minipoa: 0 after a band violation
+// (align.cpp:1584-1588), 1 after a backtrack failure (:1730-1731).
 #include <zlib.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -29,14 +32,20 @@ struct ReadSet {
 
 // FASTA / FASTQ (optionally gzip) in kseq's split: name = header up to the first blank,
 // comment = the rest; sequence lines joined (src/handle_input.cpp:13-43).
-bool load(const char* path, ReadSet& rs) {
+// A whole file, plain or gzip.
+bool slurp(const char* path, std::string* all) {
   gzFile fp = gzopen(path, "r");
   if (!fp) return false;
-  std::string all;
   char buf[1 << 16];
   int got;
-  while ((got = gzread(fp, buf, sizeof buf)) > 0) all.append(buf, (size_t)got);
+  while ((got = gzread(fp, buf, sizeof buf)) > 0) all->append(buf, (size_t)got);
   gzclose(fp);
+  return true;
+}
+
+bool load(const char* path, ReadSet& rs) {
+  std::string all;
+  if (!slurp(path, &all)) return false;
   size_t at = 0;
   auto next_line = [&](std::string& line) {
     if (at >= all.size()) return false;
@@ -92,7 +101,8 @@ void usage() {
                "  -E INT gap extend (-2) -b INT band b (100)  -f INT band f (40)\n"
                "  -B adaptive band      -S seeding           -k INT k-mer (19)\n"
                "  -w INT window (10)    -W INT sub-window     -r INT 0 consensus, 1 MSA, 2 GFA, 3 MSA + consensus row\n"
-               "  -p progressive order  -V INT verbosity      -t INT threads (ignored)\n");
+               "  -p progressive order  -V INT verbosity      -t INT threads (ignored)\n"
+               "  -i FILE align to the graph in this GFA\n");
 }
 
 int exit_status(int rc) {
@@ -101,12 +111,79 @@ int exit_status(int rc) {
   return 1;
 }
 
+// minipoa -i (src/main.cpp:86-126): the GFA's paths are the first reads, then rs's reads are
+// aligned one by one, each added and the graph sorted, and the output printed. A GFA file that
+// cannot be opened gives an empty graph, as in minipoa (handle_input.cpp:452-455).
+int incremental(mpoa_ctx* ctx, const mpoa_params& p, const char* gfa_path, const ReadSet& rs) {
+  std::string gfa;
+  if (!slurp(gfa_path, &gfa)) gfa.clear();
+  mpoa_graph* g = nullptr;
+  mpoa_seeder* mm = nullptr;
+  int rc = mpoa_graph_create(ctx, &p, &g);
+  std::string names(gfa.size() + 1, '\0');  // a path name and its newline fit in its P line
+  int64_t names_len = 0;
+  int32_t n_paths = 0;
+  if (rc == MPOA_OK)
+    rc = mpoa_graph_read_gfa(g, gfa.data(), (int64_t)gfa.size(), &names[0],
+                             (int64_t)names.size(), &names_len, &n_paths);
+  // the read set: paths (a name, no comment, no bases), then the FASTA's reads
+  std::string name(names, 0, (size_t)names_len), comment = rs.comment, seq = rs.seq;
+  name.erase(std::remove(name.begin(), name.end(), '\n'), name.end());
+  std::vector<int64_t> name_off{0}, comment_off, seq_off;
+  for (size_t a = 0, k = 0; (int)k < n_paths; k++) {
+    const size_t nl = names.find('\n', a);
+    name_off.push_back(name_off.back() + (int64_t)(nl - a));
+    a = nl + 1;
+  }
+  const int64_t path_name_len = name_off.back();
+  name += rs.name;
+  for (int r = 1; r <= rs.n(); r++) name_off.push_back(path_name_len + rs.name_off[r]);
+  comment_off.assign((size_t)n_paths, 0);
+  seq_off.assign((size_t)n_paths, 0);
+  for (int r = 0; r <= rs.n(); r++) {
+    comment_off.push_back(rs.comment_off[r]);
+    seq_off.push_back(rs.seq_off[r]);
+  }
+  const int n = n_paths + rs.n();
+  if (rc == MPOA_OK) rc = mpoa_seeder_create(&p, n, seq.data(), seq_off.data(), &mm);
+  std::vector<mpoa_res> res;
+  for (int i = n_paths; i < n && rc == MPOA_OK; i++) {
+    const int qlen = (int)(seq_off[i + 1] - seq_off[i]);
+    res.resize((size_t)(2 * qlen + 8));
+    int64_t got = 0;
+    rc = mpoa_alignment(g, mm, i, seq.data() + seq_off[i], qlen, res.data(),
+                        (int64_t)res.size(), &got);
+    if (rc == MPOA_E_OUT_CAP) {
+      res.resize((size_t)got);
+      rc = mpoa_alignment(g, mm, i, seq.data() + seq_off[i], qlen, res.data(), got, &got);
+    }
+    if (rc == MPOA_OK) rc = mpoa_graph_add_path(g, i, res.data(), got);
+    if (rc == MPOA_OK) rc = mpoa_graph_topsort(g, i + 1 == n ? 1 : 0);
+  }
+  std::string text;
+  int64_t len = 0;
+  if (rc == MPOA_OK)
+    rc = mpoa_graph_output(g, p.result, name.data(), name_off.data(), comment.data(),
+                           comment_off.data(), nullptr, nullptr, 0, &len);
+  if (rc == MPOA_OK) {
+    text.resize((size_t)std::max<int64_t>(len, 1));
+    rc = mpoa_graph_output(g, p.result, name.data(), name_off.data(), comment.data(),
+                           comment_off.data(), nullptr, &text[0], len, &len);
+  }
+  if (rc == MPOA_OK) std::fwrite(text.data(), 1, (size_t)len, stdout);
+  else std::fprintf(stderr, "minipoa_amd: %s\n", mpoa_strerror(rc));
+  mpoa_seeder_destroy(mm);
+  mpoa_graph_destroy(g);
+  return rc;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   mpoa_params p = {2, -4, -4, -2, 100, 40, 0, 0, 19, 10, 0, 0, 0};
   int verbose = 0;
   std::vector<const char*> files;
+  const char* gfa = nullptr;
   for (int i = 1; i < argc; i++) {
     const char* s = argv[i];
     if (s[0] != '-' || s[1] == '\0') {
@@ -138,6 +215,14 @@ int main(int argc, char** argv) {
       case 'r': p.result = value(); break;
       case 'V': verbose = value(); break;
       case 't': (void)value(); break;
+      case 'i':
+        if (s[2]) gfa = s + 2;
+        else if (i + 1 < argc) gfa = argv[++i];
+        else {
+          std::fprintf(stderr, "option -i needs a value\n");
+          return 1;
+        }
+        break;
       case 'h': usage(); return 0;
       default:
         std::fprintf(stderr, "unknown option -%c\n", opt);
@@ -161,6 +246,17 @@ int main(int argc, char** argv) {
   if (rc != MPOA_OK) {
     std::fprintf(stderr, "minipoa_amd: %s\n", mpoa_strerror(rc));
     return 1;
+  }
+  if (gfa) {
+    if (sets.size() != 1) {
+      std::fprintf(stderr, "minipoa_amd: -i takes one read set\n");
+      mpoa_destroy(ctx);
+      return 1;
+    }
+    p.progressive = 0;  // minipoa builds no guide tree with -i (main.cpp:101)
+    rc = incremental(ctx, p, gfa, sets[0]);
+    mpoa_destroy(ctx);
+    return exit_status(rc);
   }
   if (sets.size() == 1 || p.seeding) {
     // minipoa's own run, one read set at a time

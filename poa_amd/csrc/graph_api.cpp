@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "host.h"
@@ -193,10 +194,15 @@ int do_topsort(mpoa_graph* g, int op_code) {
 // poa() on [rank(beg_id), rank(end_id)]: appends its res entries to `res`.
 int do_poa(mpoa_graph* g, int beg_id, int end_id, const uint8_t* q, int qlen, int ab_band,
            std::vector<mpoa_res>* res, int* score) {
-  if (beg_id < 0 || end_id < 0 || beg_id >= g->st.n_nodes || end_id >= g->st.n_nodes ||
-      qlen < 0) {
+  if (beg_id < 0 || end_id < 0 || beg_id >= g->st.n_nodes || end_id >= g->st.n_nodes) {
     g_err = "node id out of range";
     return MPOA_E_ARG;
+  }
+  if (qlen < 0) {
+    // a -S sub-window that ends before it begins in the read (overlapping anchors); minipoa's
+    // vector of negative size aborts there
+    g_err = "query sub-window of negative length";
+    return MPOA_E_EMPTY_WINDOW;
   }
   if (g->sorted != 0) {
     g_err = "the graph must be sorted (topsort op 0) after its last add_path";
@@ -470,6 +476,127 @@ int mpoa_graph_add_path(mpoa_graph* g, int32_t seq_id, const mpoa_res* res, int6
   g->read_len.push_back((int)len);
   g->path0 += len;
   g->sorted = -1;
+  return MPOA_OK;
+}
+
+int mpoa_graph_read_gfa(mpoa_graph* g, const char* text, int64_t len, char* names,
+                        int64_t names_cap, int64_t* names_len, int32_t* n_paths) {
+  if (!g || len < 0 || (len > 0 && !text)) return MPOA_E_ARG;
+  if (g->st.n_nodes != 2 || !g->read_len.empty()) {
+    g_err = "a GFA graph is read into a graph that holds only the source and the sink";
+    return MPOA_E_ARG;
+  }
+  if (g->params.result == 2) {
+    // minipoa's GFA output walks read_paths, which read_gfa leaves empty for the GFA's paths
+    g_err = "GFA input with GFA output (-i with -r 2) is not supported";
+    return MPOA_E_ARG;
+  }
+  HIPCHK(hipSetDevice(g->ctx->device));
+  // Lines in order (handle_input.cpp:455-479): S adds a segment's bases as new nodes
+  // (gfa_parse_S, :78-127), P adds a path over existing segments (gfa_parse_P, :128-261);
+  // other lines are skipped. Nodes come first on the device (a path only joins existing nodes).
+  struct Seg { int in, out; };
+  std::unordered_map<std::string, Seg> segs;
+  std::vector<uint8_t> bases;  // node 2 + k has base bases[k]
+  std::vector<std::string> path_names;
+  std::vector<std::vector<int>> paths;
+  int next = 2;
+  int64_t at = 0;
+  while (at < len) {
+    const char* nl = static_cast<const char*>(std::memchr(text + at, '\n', (size_t)(len - at)));
+    const int64_t end = nl ? nl - text : len;
+    std::string line(text + at, (size_t)(end - at));
+    at = end + 1;
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.size() < 3 || line[1] != '\t') continue;
+    std::vector<std::string> f;
+    for (size_t a = 2;;) {
+      const size_t t = line.find('\t', a);
+      f.push_back(line.substr(a, t == std::string::npos ? std::string::npos : t - a));
+      if (t == std::string::npos) break;
+      a = t + 1;
+    }
+    if (line[0] == 'S') {
+      if (f.size() < 2 || f[1].empty()) {
+        g_err = "GFA segment line without a sequence: " + f[0];
+        return MPOA_E_ARG;
+      }
+      if (segs.count(f[0])) {
+        g_err = "duplicated GFA segment: " + f[0];
+        return MPOA_E_ARG;
+      }
+      segs[f[0]] = Seg{next, next + (int)f[1].size() - 1};
+      for (char c : f[1]) bases.push_back(code_of_char((unsigned char)c));
+      next += (int)f[1].size();
+    } else if (line[0] == 'P') {
+      if (f.size() < 2) {
+        g_err = "GFA path line without a path: " + f[0];
+        return MPOA_E_ARG;
+      }
+      std::vector<int> nodes;
+      const std::string& w = f[1];
+      size_t a = 0;
+      for (size_t k = 0; k < w.size(); k++) {
+        if (w[k] == '-') {
+          g_err = "reverse-strand GFA path (" + f[0] + "): only '+' segments are read";
+          return MPOA_E_ARG;
+        }
+        if (w[k] != '+') continue;
+        auto it = segs.find(w.substr(a, k - a));
+        if (it == segs.end()) {
+          g_err = "GFA path " + f[0] + " names an unknown segment: " + w.substr(a, k - a);
+          return MPOA_E_ARG;
+        }
+        for (int v = it->second.in; v <= it->second.out; v++) nodes.push_back(v);
+        a = k + 2;  // past "+,"
+      }
+      if (nodes.empty()) {
+        g_err = "GFA path without segments: " + f[0];
+        return MPOA_E_ARG;
+      }
+      path_names.push_back(f[0]);
+      paths.push_back(std::move(nodes));
+    }
+  }
+  int rc;
+  if (!bases.empty()) {
+    const int cnt = (int)bases.size();
+    if (2 + cnt > g->caps.nodes) {
+      SlotCaps nc = g->caps;
+      nc.nodes = 2 + cnt + cnt / 2;
+      nc.edges = std::max(g->caps.edges, 2 * (2 + cnt));
+      if ((rc = regrow(g, nc)) != MPOA_OK) return rc;
+    }
+    if (!g->d_q.p || g->d_q.n < (size_t)cnt)
+      if ((rc = g->d_q.alloc((size_t)cnt)) != MPOA_OK) return rc;
+    HIPCHK(hipMemcpy(g->d_q.p, bases.data(), (size_t)cnt, hipMemcpyHostToDevice));
+    GraphOpArgs op{};
+    op.op = mpoa::GOP_ADD_NODES;
+    op.q = g->d_q.p;
+    op.qlen = cnt;
+    GraphOpOut out{};
+    if ((rc = run_op(g, op, &out)) != MPOA_OK) return rc;
+    if ((rc = check_status(out.status)) != MPOA_OK) return rc;
+  }
+  // each path: the source, its nodes, the sink, as a res of existing nodes (add_adj per step,
+  // positions counted from 0: handle_input.cpp:196-205, 248-250)
+  for (size_t k = 0; k < paths.size(); k++) {
+    std::vector<mpoa_res> res;
+    res.push_back(mpoa_res{0, 4, -1});
+    for (int v : paths[k]) res.push_back(mpoa_res{v, bases[(size_t)(v - 2)], -1});
+    if ((rc = mpoa_graph_add_path(g, (int32_t)g->read_len.size(), res.data(),
+                                  (int64_t)res.size())) != MPOA_OK)
+      return rc;
+  }
+  if ((rc = do_topsort(g, 0)) != MPOA_OK) return rc;  // handle_input.cpp:482
+  std::string all;
+  for (const std::string& nm : path_names) all += nm + "\n";
+  if (n_paths) *n_paths = (int32_t)path_names.size();
+  if (names_len) *names_len = (int64_t)all.size();
+  if (names) {
+    if ((int64_t)all.size() > names_cap) return MPOA_E_OUT_CAP;
+    std::memcpy(names, all.data(), all.size());
+  }
   return MPOA_OK;
 }
 

@@ -1,7 +1,7 @@
 // minipoa's main loop (src/main.cpp:84-126) written against the C++ facade
 // include/minipoa_amd.hpp: the program a minipoa user keeps, with the alignment path on the
-// GPU. Reads one plain FASTA file; flags -M -X -O -E -b -f -B -S -k -w -p -r as minipoa's
-// (src/main.cpp:14-35). Prints what minipoa prints.
+// GPU. Reads one plain FASTA file and, with -i, a plain GFA to align it to; flags -M -X -O -E
+// -b -f -B -S -k -w -p -r -i as minipoa's (src/main.cpp:14-35). Prints what minipoa prints.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -38,6 +38,7 @@ static std::vector<seq_t> read_fasta(const char* path) {
 int main(int argc, char** argv) {
   para_t* para = new para_t();
   const char* path = nullptr;
+  const char* inc = nullptr;
   for (int i = 1; i < argc; i++) {
     const std::string a = argv[i];
     auto val = [&]() { return i + 1 < argc ? std::atoi(argv[++i]) : 0; };
@@ -53,19 +54,23 @@ int main(int argc, char** argv) {
     else if (a == "-k") para->k = val();
     else if (a == "-w") para->mm_w = val();
     else if (a == "-r") para->result = val();
+    else if (a == "-i") inc = i + 1 < argc ? argv[++i] : nullptr;
     else path = argv[i];
   }
   if (!path) {
-    std::cerr << "usage: minipoa_loop [flags] reads.fa\n";
+    std::cerr << "usage: minipoa_loop [flags] [-i graph.gfa] reads.fa\n";
     return 1;
   }
   try {
-    std::vector<seq_t> seqs = read_fasta(path);
     graph* DAG = new graph();
     DAG->init(para);
+    std::vector<seq_t> seqs;
+    if (inc) seqs = read_gfa(para, DAG, inc);
+    const int exist_seq_num = (int)seqs.size();
+    for (seq_t& s : read_fasta(path)) seqs.push_back(std::move(s));
     minimizer_t* mm = new minimizer_t(para, seqs);
-    mm->get_guide_tree(para);
-    for (int i = 0; i < (int)seqs.size(); i++) {
+    if (!inc) mm->get_guide_tree(para);
+    for (int i = exist_seq_num; i < (int)seqs.size(); i++) {
       const int rid = mm->ord[i];
       std::vector<res_t> res =
           alignment(para, DAG, mm, rid, seqs[rid].seq.c_str(), (int)seqs[rid].seq.size());
@@ -75,6 +80,7 @@ int main(int argc, char** argv) {
     if (para->result == 0) DAG->output_consensus();
     else if (para->result == 1) DAG->output_rc_msa(para, mm->rid_to_ord, seqs);
     else if (para->result == 2) DAG->output_gfa(mm->rid_to_ord, seqs);
+    else if (para->result == 3) std::cout << DAG->text(3, &seqs, &mm->rid_to_ord);
     delete mm;
     delete DAG;
   } catch (const error& e) {

@@ -197,7 +197,10 @@ struct SeedArgs {
 // One step of minipoa's per-read loop on a graph kept in device memory between calls (the
 // call-level API: poa(), graph::add_path, graph::topsort, the outputs). The graph is slot 0 of
 // a's pools, its counters and status in `state`.
-enum GraphOp { GOP_INIT = 0, GOP_POA = 1, GOP_ADD_PATH = 2, GOP_TOPSORT = 3, GOP_OUTPUT = 4 };
+enum GraphOp {
+  GOP_INIT = 0, GOP_POA = 1, GOP_ADD_PATH = 2, GOP_TOPSORT = 3, GOP_OUTPUT = 4,
+  GOP_ADD_NODES = 5  // graph::add_node for each of q[0..qlen) (codes 0..4), without edges
+};
 
 struct GraphOpOut {
   int status, score, rows, qlen, width, pad;

@@ -2599,6 +2599,29 @@ __global__ __launch_bounds__(NT) void k_graph_op(GraphOpArgs g) {
       }
       __syncthreads();
     }
+  } else if (g.op == GOP_ADD_NODES) {
+    // graph::add_node per segment base (graph.cpp:7-12, read_gfa's S lines,
+    // handle_input.cpp:103-121): ids in order, each its own column head, no edges yet
+    const int n0 = S.n_nodes, cnt = g.qlen;
+    if (S.status == MPOA_OK && n0 + cnt <= a.caps.nodes) {
+      for (int k = tid; k < cnt; k += NT) {
+        const int v = n0 + k, b = g.q[k];
+        A.base[v] = (uint8_t)b;
+        A.par[v] = v;
+        for (int c = 0; c < NB; c++) A.aligned[v * NB + c] = -1;
+        A.aligned[v * NB + b] = v;
+        A.ind[v] = 0;
+        A.in_head[v] = A.in_tail[v] = A.out_head[v] = A.out_tail[v] = -1;
+        A.in_cnt[v] = 0;
+        A.out_cnt[v] = 0;
+        A.last_read[v] = -1;
+      }
+      __syncthreads();
+      if (tid == 0) S.n_nodes = n0 + cnt;
+    } else if (tid == 0 && S.status == MPOA_OK) {
+      S.status = MPOA_E_OVERFLOW;
+    }
+    __syncthreads();
   } else if (g.op == GOP_TOPSORT) {
     // graph::topsort(para, op) (graph.cpp:21-128); with seeding, build_consensus
     if (S.status == MPOA_OK) {

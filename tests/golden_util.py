@@ -88,3 +88,13 @@ def check_msa_consensus_row(text3: str, text_r1: str, text_r0: str) -> None:
         if ch != "-":
             assert any(r[c] == ch for r in rows), f"column {c}"
 
+
+
+def gfa_read_set(gfa_text: str, fasta_text: str):
+    """minipoa -i's read set (src/main.cpp:86-89): the GFA's P lines first, as reads with a name,
+    no comment and no bases, then the FASTA's reads. Returns (seq, off, names, comments,
+    number of paths)."""
+    paths = [ln.split("\t")[1] for ln in gfa_text.splitlines() if ln.startswith("P\t")]
+    seq, off, names, comments = parse_fasta(fasta_text)
+    off = np.concatenate([np.zeros(len(paths), np.int64), off])
+    return seq, off, paths + names, [""] * len(paths) + comments, len(paths)

@@ -196,16 +196,21 @@ class Dag:
         lib().orc_dag_order(ctypes.c_void_p(self.h), _p(out), n)
         return out[:n]
 
-    def _res(self, fn, *args) -> np.ndarray:
+    def _res(self, fn, *args, guess: int = 64) -> np.ndarray:
+        """One call with room for `guess` entries; a second, sized one only when that was too
+        small (the call is computed again then)."""
         n = ctypes.c_int64()
-        rc = fn(ctypes.c_void_p(self.h), *args, None, ctypes.c_int64(0), ctypes.byref(n))
-        if rc != 0:
-            raise RuntimeError(f"oracle error {rc}")
-        out = np.zeros((max(n.value, 1), 3), np.int32)
+        out = np.zeros((max(guess, 1), 3), np.int32)
         rc = fn(ctypes.c_void_p(self.h), *args, _p(out), ctypes.c_int64(len(out)),
                 ctypes.byref(n))
         if rc != 0:
             raise RuntimeError(f"oracle error {rc}")
+        if n.value > len(out):
+            out = np.zeros((n.value, 3), np.int32)
+            rc = fn(ctypes.c_void_p(self.h), *args, _p(out), ctypes.c_int64(len(out)),
+                    ctypes.byref(n))
+            if rc != 0:
+                raise RuntimeError(f"oracle error {rc}")
         return out[:n.value]
 
     def poa(self, beg_id: int, end_id: int, q: bytes, ab_band: bool):
@@ -222,7 +227,7 @@ class Dag:
 
     def alignment(self, rid: int, q: bytes) -> np.ndarray:
         buf = ctypes.create_string_buffer(q, len(q) + 1)
-        return self._res(lib().orc_dag_alignment, rid, buf, len(q))
+        return self._res(lib().orc_dag_alignment, rid, buf, len(q), guess=2 * len(q) + 8)
 
     def add_path(self, seq_id: int, res: np.ndarray) -> None:
         res = np.ascontiguousarray(res, np.int32)
@@ -234,6 +239,25 @@ class Dag:
     def topsort(self, op: int) -> None:
         if lib().orc_dag_topsort(ctypes.c_void_p(self.h), op) != 0:
             raise RuntimeError("oracle topsort failed")
+
+    def scores(self) -> list[int]:
+        """The scores of the DP calls completed on this graph, in order (-V 1's lines)."""
+        n = lib().orc_dag_scores(ctypes.c_void_p(self.h), None, 0)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_dag_scores(ctypes.c_void_p(self.h), _p(out), n)
+        return [int(x) for x in out[:n]]
+
+    def read_gfa(self, text: bytes) -> list[str]:
+        """read_gfa (minipoa -i) into this new graph; returns the path names (a name and its
+        newline take no more room than its P line)."""
+        buf = ctypes.create_string_buffer(text, len(text) + 1)
+        out = ctypes.create_string_buffer(len(text) + 1)
+        n, k = ctypes.c_int64(), ctypes.c_int32()
+        rc = lib().orc_dag_read_gfa(ctypes.c_void_p(self.h), buf, ctypes.c_int64(len(text)), out,
+                                    ctypes.c_int64(len(out)), ctypes.byref(n), ctypes.byref(k))
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        return out.raw[:n.value].decode().split("\n")[:k.value]
 
     def output(self, result: int, names=None, comments=None) -> str:
         nb, no = _names(names)
@@ -275,17 +299,24 @@ class Dag:
 
 
 def run_loop(dag, n_reads: int, seq: np.ndarray, off: np.ndarray, result: int,
-             names=None, comments=None, seeder=None, order=None):
-    """minipoa's main loop (src/main.cpp:100-126) driven through a call-level object (the
-    oracle's Dag or the product's api.Graph): per read, in `order` (the guide tree with -p),
-    alignment -> add_path -> topsort (op 1 after the last read), then the output, whose MSA
-    rows map back through the order. Returns (text, [res of every read])."""
+             names=None, comments=None, seeder=None, order=None, gfa: bytes | None = None):
+    """minipoa's main loop (src/main.cpp:86-126) driven through a call-level object (the
+    oracle's Dag or the product's api.Graph): with `gfa`, read_gfa first, whose paths are the
+    first reads of the set (seq/off hold them as empty reads, names their names); then per read,
+    in `order` (the guide tree with -p), alignment -> add_path -> topsort (op 1 after the last
+    read), then the output, whose MSA rows map back through the order. Returns (text, [res of
+    every read aligned])."""
     ord_ = list(range(n_reads)) if order is None else [int(x) for x in order]
     rid_to_ord = np.zeros(n_reads, np.int32)
     for i, r in enumerate(ord_):
         rid_to_ord[r] = i
+    first = 0
+    if gfa is not None:
+        first = len(dag.read_gfa(gfa))
     all_res = []
     for i, rid in enumerate(ord_):
+        if i < first:
+            continue
         q = bytes(seq[off[rid]:off[rid + 1]])
         res = dag.alignment(rid, q) if seeder is None else dag.alignment(rid, q, seeder)
         all_res.append(res)

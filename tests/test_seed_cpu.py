@@ -84,3 +84,26 @@ def test_guide_tree_order_matches_oracle():
         got = api.Seeder(api.Params(**p), seq, off).order()
         assert list(got) == list(want)
         assert sorted(got) == list(range(len(off) - 1))
+
+
+def test_seeding_with_gfa_paths_equals_oracle():
+    """minipoa -i's read set starts with the GFA's paths as reads without bases (src/main.cpp:86,
+    minimizers over all of them at :98): the reads after them seed as in the oracle."""
+    import gzip
+    import json
+    import os
+
+    import golden_util as gu
+    fix = json.load(gzip.open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                           "gfa_input.json.gz"), "rt"))
+    total = 0
+    for inp in fix["inputs"].values():
+        seq, off, _, _, n_paths = gu.gfa_read_set(inp["gfa"], inp["fasta"])
+        p = api.Params(seeding=1)
+        for rid in range(n_paths, len(off) - 1):
+            cons = bytes(seq[off[n_paths]:off[n_paths + 1]])
+            a = _anchors_product(p, seq, off, rid, cons)
+            b = _anchors_oracle(p, seq, off, rid, cons)
+            assert np.array_equal(a, b), rid
+            total += len(a)
+    assert total > 0
