@@ -172,6 +172,44 @@ def single_window(ctx, cores: int, with_cpu: bool, log) -> dict:
     return out
 
 
+def ingest(batch, threads: int, log) -> dict:
+    """Read-set ingestion as a caller with files would do it: every window of the workload
+    written as its own FASTA file (60-column lines), then all files parsed and packed by the
+    library (mpoa_reads_load, minipoa's readFile rules) on `threads` host threads. The packed
+    arrays must equal the workload's."""
+    from poa_amd import api
+    tmp = tempfile.mkdtemp(prefix="mpoa_ingest_")
+    try:
+        paths = []
+        nbytes = 0
+        for w in range(len(batch.win_off) - 1):
+            r0, r1 = int(batch.win_off[w]), int(batch.win_off[w + 1])
+            parts = []
+            for r in range(r0, r1):
+                s = batch.seq[batch.seq_off[r]:batch.seq_off[r + 1]].tobytes()
+                parts.append(b">r%d window %d\n" % (r - r0, w))
+                parts.extend(s[k:k + 60] + b"\n" for k in range(0, len(s), 60))
+            data = b"".join(parts)
+            path = os.path.join(tmp, f"w{w:06d}.fa")
+            with open(path, "wb") as f:
+                f.write(data)
+            paths.append(path)
+            nbytes += len(data)
+        rs = api.ReadSet(paths, threads=threads)
+        dt = rs.load_s
+        same = (np.array_equal(rs.win_off, batch.win_off) and
+                np.array_equal(rs.seq_off, batch.seq_off) and
+                np.array_equal(rs.seq, batch.seq[:int(batch.seq_off[-1])]))
+        log(f"ingest: {len(paths)} files, {nbytes / 1e6:.1f} MB parsed and packed in "
+            f"{dt:.3f} s on {threads} threads, identical {same}")
+        return {"files": len(paths), "bytes": nbytes, "threads": threads, "s": round(dt, 3),
+                "windows_per_s": round(len(paths) / dt, 1), "MB_per_s": round(nbytes / dt / 1e6, 1),
+                "arrays_identical": bool(same),
+                "note": "file text to packed arrays (mpoa_reads_load); the upload is prepare_s"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def estimate_form(ctx, params, batch, steps: int, log) -> dict:
     """The same batch scheduled by the read-length estimate (no network): windows/s, device
     memory of the arenas and overflow re-runs, for the A/B against the learned schedule."""
@@ -244,6 +282,8 @@ def main() -> int:
                     help="windows run in the plain one-window-per-launch form (0: skip)")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-window configurations")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip the timing of read-set ingestion from FASTA files")
     ap.add_argument("--no-ab", action="store_true",
                     help="skip the learned-against-estimate scheduling A/B")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -432,6 +472,8 @@ def main() -> int:
                         "reruns": int(st.reruns)},
             "read_length_estimate": {k: v for k, v in est.items() if k != "texts"},
             "outputs_identical": est["texts"] == res.texts}
+    if not args.no_ingest:
+        line["ingest"] = ingest(batch, cores, log)
     if not args.no_single:
         line["single_window"] = single_window(ctx, cores, not args.no_cpu, log)
     ctx.close()

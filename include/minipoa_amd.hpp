@@ -252,6 +252,28 @@ struct graph {
   }
 };
 
+// readFile (src/handle_input.cpp:13-43): the FASTA/FASTQ(.gz) reads of `path` appended to seqs,
+// parsed by the library with kseq's rules. A file that cannot be opened throws, as minipoa's
+// readFile throws std::runtime_error.
+inline void readFile(std::vector<seq_t>& seqs, const char* path) {
+  mpoa_reads* r = nullptr;
+  if (mpoa_reads_load(&path, 1, 1, &r) != MPOA_OK)
+    throw std::runtime_error(std::string("Error opening file ") + path);
+  int32_t n = 0;
+  const char *seq, *name, *comment;
+  const int64_t *seq_off, *name_off, *comment_off;
+  mpoa_reads_view(r, nullptr, nullptr, &n, &seq, &seq_off, &name, &name_off, &comment,
+                  &comment_off);
+  for (int i = 0; i < n; i++) {
+    seq_t s;
+    s.name.assign(name + name_off[i], (size_t)(name_off[i + 1] - name_off[i]));
+    s.comment.assign(comment + comment_off[i], (size_t)(comment_off[i + 1] - comment_off[i]));
+    s.seq.assign(seq + seq_off[i], (size_t)(seq_off[i + 1] - seq_off[i]));
+    seqs.push_back(std::move(s));
+  }
+  mpoa_reads_free(r);
+}
+
 // read_gfa (src/handle_input.cpp:444-495, minipoa's -i): the plain-text GFA at `path` into DAG,
 // fresh from init; returns the paths as the read set's first seq_t entries. A file that cannot
 // be opened gives no entries, as in minipoa (:452-455).

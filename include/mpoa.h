@@ -221,6 +221,30 @@ int mpoa_align_reads_noalloc(mpoa_ctx* ctx, const mpoa_params* params, int32_t n
                              char* d_out, int64_t out_cap, int64_t* d_res, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Read sets: readFile (src/handle_input.cpp:13-43: kseq_read over gzread, FASTA or FASTQ,
+ * plain or gzip) as a library call, for one file or many. Each file is one window; files are
+ * parsed on n_threads host threads (0: one per hardware thread) and packed into the arrays
+ * mpoa_align_reads and mpoa_batch_prepare take. Names, comments and bases follow kseq's rules
+ * exactly (the name up to the first white space, the comment the rest of the header line,
+ * sequence lines joined as they are); a FASTQ record whose quality is cut short ends its file
+ * with the records before it, as readFile's loop stops there. A file that cannot be opened is
+ * MPOA_E_ARG (minipoa: "Error opening file").
+ * ------------------------------------------------------------------------------------ */
+typedef struct mpoa_reads mpoa_reads;
+int mpoa_reads_load(const char* const* paths, int32_t n_files, int32_t n_threads,
+                    mpoa_reads** out);
+/* One window from a text in memory. */
+int mpoa_reads_parse(const char* text, int64_t len, mpoa_reads** out);
+void mpoa_reads_free(mpoa_reads* r);
+/* The packed arrays, valid until mpoa_reads_free (any pointer may be NULL): n_windows and
+ * win_off (n_windows + 1 read offsets), n_reads, seq with seq_off (n_reads + 1 byte offsets),
+ * name/name_off, comment/comment_off laid out the same way. */
+int mpoa_reads_view(const mpoa_reads* r, int32_t* n_windows, const int32_t** win_off,
+                    int32_t* n_reads, const char** seq, const int64_t** seq_off,
+                    const char** name, const int64_t** name_off, const char** comment,
+                    const int64_t** comment_off);
+
+/* ------------------------------------------------------------------------------------
  * Batched windows. minipoa has no batch interface: it aligns one read set per process
  * (src/main.cpp:110-121). A batch is many independent read sets ("windows"), each the
  * equivalent of one minipoa run, all processed by one persistent device launch.

@@ -34,7 +34,8 @@ EXPORTS = [
     "mpoa_poa", "mpoa_seeder_create", "mpoa_seeder_destroy", "mpoa_seeder_order",
     "mpoa_alignment",
     "mpoa_graph_add_path", "mpoa_graph_topsort", "mpoa_graph_output", "mpoa_graph_export",
-    "mpoa_graph_read_gfa", "mpoa_batch_set_predictions",
+    "mpoa_graph_read_gfa", "mpoa_reads_load", "mpoa_reads_parse", "mpoa_reads_free",
+    "mpoa_reads_view", "mpoa_batch_set_predictions",
     "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
     "mpoa_mlp_train", "mpoa_mlp_error",
@@ -143,6 +144,12 @@ def lib() -> ctypes.CDLL:
         L.mpoa_graph_add_path.argtypes = [VP, I32, VP, I64]
         L.mpoa_graph_topsort.argtypes = [VP, I32]
         L.mpoa_graph_read_gfa.argtypes = [VP, VP, I64, VP, I64, P(I64), P(I32)]
+        L.mpoa_reads_load.argtypes = [VP, I32, I32, P(VP)]
+        L.mpoa_reads_parse.argtypes = [VP, I64, P(VP)]
+        L.mpoa_reads_free.argtypes = [VP]
+        L.mpoa_reads_free.restype = None
+        L.mpoa_reads_view.argtypes = [VP, P(I32), P(VP), P(I32), P(VP), P(VP), P(VP), P(VP),
+                                      P(VP), P(VP)]
         L.mpoa_graph_output.argtypes = [VP, I32, VP, VP, VP, VP, VP, VP, I64, P(I64)]
         L.mpoa_seeder_order.argtypes = [VP, VP, I32]
         L.mpoa_seeder_order.restype = I32
@@ -352,6 +359,67 @@ class Batch:
 # on a graph kept in device memory (include/mpoa.h, "Call-level API").
 # ---------------------------------------------------------------------------------------
 RES_DTYPE = np.dtype([("from", np.int32), ("base", np.int32), ("aligned_id", np.int32)])
+
+
+class ReadSet:
+    """minipoa's readFile for one or many FASTA/FASTQ(.gz) files (one window each), parsed by
+    the library on host threads: packed arrays ready for Context.align_reads and Batch."""
+
+    def __init__(self, paths: list[str] | None = None, threads: int = 0,
+                 text: bytes | None = None):
+        import time
+        h = ctypes.c_void_p()
+        if text is not None:
+            buf = ctypes.create_string_buffer(text, len(text) + 1)
+            t0 = time.perf_counter()
+            _check(lib().mpoa_reads_parse(buf, len(text), ctypes.byref(h)))
+        else:
+            enc = [os.fsencode(p) for p in paths]
+            arr = (ctypes.c_char_p * max(len(enc), 1))(*enc)
+            t0 = time.perf_counter()
+            _check(lib().mpoa_reads_load(arr, len(enc), threads, ctypes.byref(h)))
+        self.load_s = time.perf_counter() - t0  # the library call alone
+        try:
+            nw, nr = ctypes.c_int32(), ctypes.c_int32()
+            ptr = [ctypes.c_void_p() for _ in range(7)]
+            _check(lib().mpoa_reads_view(h, ctypes.byref(nw), ctypes.byref(ptr[0]),
+                                         ctypes.byref(nr), *[ctypes.byref(x) for x in ptr[1:]]))
+
+            def arr_of(p, n, dt):
+                if n == 0 or not p.value:
+                    return np.zeros(n, dt)
+                return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(
+                    np.ctypeslib.as_ctypes_type(dt))), (n,)).copy()
+
+            self.win_off = arr_of(ptr[0], nw.value + 1, np.int32)
+            self.seq_off = arr_of(ptr[2], nr.value + 1, np.int64)
+            self.name_off = arr_of(ptr[4], nr.value + 1, np.int64)
+            self.comment_off = arr_of(ptr[6], nr.value + 1, np.int64)
+            self.seq = arr_of(ptr[1], int(self.seq_off[-1]), np.uint8)
+            self.name = arr_of(ptr[3], int(self.name_off[-1]), np.uint8).tobytes()
+            self.comment = arr_of(ptr[5], int(self.comment_off[-1]), np.uint8).tobytes()
+        finally:
+            lib().mpoa_reads_free(h)
+
+    @staticmethod
+    def _split(text: bytes, off: np.ndarray) -> list[str]:
+        return [text[off[i]:off[i + 1]].decode("latin-1") for i in range(len(off) - 1)]
+
+    @property
+    def names(self) -> list[str]:
+        return self._split(self.name, self.name_off)
+
+    @property
+    def comments(self) -> list[str]:
+        return self._split(self.comment, self.comment_off)
+
+    @property
+    def n_windows(self) -> int:
+        return len(self.win_off) - 1
+
+    @property
+    def n_reads(self) -> int:
+        return len(self.seq_off) - 1
 
 
 class Seeder:

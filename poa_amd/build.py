@@ -31,7 +31,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "graph_api.cpp", "sched.cpp", "seed.cpp"]
+PRODUCT_SOURCES = ["pipeline.hip", "mlp.hip", "mpoa.cpp", "graph_api.cpp", "sched.cpp", "seed.cpp",
+                   "reads.cpp"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:
@@ -57,7 +58,7 @@ def build_product() -> str:
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, out)
     return out
 

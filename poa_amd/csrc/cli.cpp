@@ -30,8 +30,6 @@ struct ReadSet {
   int n() const { return (int)seq_off.size() - 1; }
 };
 
-// FASTA / FASTQ (optionally gzip) in kseq's split: name = header up to the first blank,
-// comment = the rest; sequence lines joined (src/handle_input.cpp:13-43).
 // A whole file, plain or gzip.
 bool slurp(const char* path, std::string* all) {
   gzFile fp = gzopen(path, "r");
@@ -43,55 +41,24 @@ bool slurp(const char* path, std::string* all) {
   return true;
 }
 
-bool load(const char* path, ReadSet& rs) {
-  std::string all;
-  if (!slurp(path, &all)) return false;
-  size_t at = 0;
-  auto next_line = [&](std::string& line) {
-    if (at >= all.size()) return false;
-    size_t e = all.find('\n', at);
-    if (e == std::string::npos) e = all.size();
-    line.assign(all, at, e - at);
-    if (!line.empty() && line.back() == '\r') line.pop_back();
-    at = e + 1;
-    return true;
-  };
-  std::string line;
-  bool have = false;
-  std::string head, body;
-  bool fastq = false;
-  auto flush = [&]() {
-    if (!have) return;
-    const size_t sp = head.find_first_of(" \t");
-    rs.name += sp == std::string::npos ? head : head.substr(0, sp);
-    rs.comment += sp == std::string::npos ? std::string() : head.substr(sp + 1);
-    rs.seq += body;
+// Window w of a loaded read set (mpoa_reads_load), copied out with offsets from 0.
+ReadSet window_of(const mpoa_reads* r, int w) {
+  const int32_t* win_off;
+  const char *seq, *name, *comment;
+  const int64_t *seq_off, *name_off, *comment_off;
+  mpoa_reads_view(r, nullptr, &win_off, nullptr, &seq, &seq_off, &name, &name_off, &comment,
+                  &comment_off);
+  ReadSet rs;
+  const int r0 = win_off[w], r1 = win_off[w + 1];
+  for (int i = r0; i < r1; i++) {
+    rs.seq.append(seq + seq_off[i], (size_t)(seq_off[i + 1] - seq_off[i]));
+    rs.name.append(name + name_off[i], (size_t)(name_off[i + 1] - name_off[i]));
+    rs.comment.append(comment + comment_off[i], (size_t)(comment_off[i + 1] - comment_off[i]));
     rs.seq_off.push_back((int64_t)rs.seq.size());
     rs.name_off.push_back((int64_t)rs.name.size());
     rs.comment_off.push_back((int64_t)rs.comment.size());
-  };
-  while (next_line(line)) {
-    if (line.empty()) continue;
-    if (line[0] == '>' || line[0] == '@') {
-      flush();
-      have = true;
-      fastq = line[0] == '@';
-      head = line.substr(1);
-      body.clear();
-      continue;
-    }
-    if (!have) continue;
-    if (fastq && line[0] == '+') {
-      // skip as many quality characters as there are bases
-      size_t q = 0;
-      while (q < body.size() && next_line(line)) q += line.size();
-      continue;
-    }
-    for (char c : line)
-      if (c != ' ' && c != '\t') body.push_back(c);
   }
-  flush();
-  return true;
+  return rs;
 }
 
 void usage() {
@@ -234,13 +201,15 @@ int main(int argc, char** argv) {
     usage();
     return 1;
   }
-  std::vector<ReadSet> sets(files.size());
-  for (size_t f = 0; f < files.size(); f++) {
-    if (!load(files[f], sets[f])) {
-      std::fprintf(stderr, "error read file: cannot open %s\n", files[f]);
-      return 1;
-    }
+  // readFile for every file (src/handle_input.cpp:13-43), parsed by the library on host threads
+  mpoa_reads* loaded = nullptr;
+  if (mpoa_reads_load(files.data(), (int32_t)files.size(), 0, &loaded) != MPOA_OK) {
+    std::fprintf(stderr, "error read file: %s\n", mpoa_strerror(MPOA_E_ARG));
+    return 1;
   }
+  std::vector<ReadSet> sets(files.size());
+  for (size_t f = 0; f < files.size(); f++) sets[f] = window_of(loaded, (int)f);
+  mpoa_reads_free(loaded);
   mpoa_ctx* ctx = nullptr;
   int rc = mpoa_create(0, &ctx);
   if (rc != MPOA_OK) {

@@ -1,6 +1,6 @@
 // minipoa's main loop (src/main.cpp:84-126) written against the C++ facade
 // include/minipoa_amd.hpp: the program a minipoa user keeps, with the alignment path on the
-// GPU. Reads one plain FASTA file and, with -i, a plain GFA to align it to; flags -M -X -O -E
+// GPU. Reads one FASTA/FASTQ(.gz) file and, with -i, a plain GFA to align it to; flags -M -X -O -E
 // -b -f -B -S -k -w -p -r -i as minipoa's (src/main.cpp:14-35). Prints what minipoa prints.
 #include <cstdio>
 #include <cstdlib>
@@ -13,27 +13,6 @@
 #include "minipoa_amd.hpp"
 
 using namespace minipoa_amd;
-
-static std::vector<seq_t> read_fasta(const char* path) {
-  std::ifstream in(path);
-  if (!in) throw std::runtime_error(std::string("cannot open ") + path);
-  std::vector<seq_t> seqs;
-  std::string line;
-  while (std::getline(in, line)) {
-    if (!line.empty() && line.back() == '\r') line.pop_back();
-    if (line.empty()) continue;
-    if (line[0] == '>') {
-      seq_t s;
-      const size_t sp = line.find_first_of(" \t");
-      s.name = line.substr(1, sp == std::string::npos ? std::string::npos : sp - 1);
-      if (sp != std::string::npos) s.comment = line.substr(sp + 1);
-      seqs.push_back(s);
-    } else if (!seqs.empty()) {
-      seqs.back().seq += line;
-    }
-  }
-  return seqs;
-}
 
 int main(int argc, char** argv) {
   para_t* para = new para_t();
@@ -67,7 +46,7 @@ int main(int argc, char** argv) {
     std::vector<seq_t> seqs;
     if (inc) seqs = read_gfa(para, DAG, inc);
     const int exist_seq_num = (int)seqs.size();
-    for (seq_t& s : read_fasta(path)) seqs.push_back(std::move(s));
+    readFile(seqs, path);
     minimizer_t* mm = new minimizer_t(para, seqs);
     if (!inc) mm->get_guide_tree(para);
     for (int i = exist_seq_num; i < (int)seqs.size(); i++) {

@@ -1,7 +1,10 @@
 """The command-line tool (poa_amd/minipoa_amd, minipoa's flags) on the golden fixtures: same
-stdout and the same -V 1 score lines as upstream minipoa; several files as one batch."""
+stdout and the same -V 1 score lines as upstream minipoa; several files as one batch; files that
+exercise the reader."""
 from __future__ import annotations
 
+import gzip
+import json
 import os
 import subprocess
 
@@ -42,3 +45,19 @@ def test_cli_batch_of_files(tmp_path):
     p = subprocess.run([CLI, *paths, "-r", "0"], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
     assert p.stdout == "".join(c["stdout"] for c in sel)
+
+
+READER = json.load(open(os.path.join(ROOT, "tests", "golden", "reader_cases.json")))["cases"]
+
+
+@pytest.mark.parametrize("case", READER, ids=lambda c: c["case"])
+def test_cli_reader_cases(case, tmp_path):
+    """Files that exercise readFile's parsing (CRLF, blanks, FASTQ, gzip, a cut-short record):
+    the same MSA text as upstream."""
+    path = tmp_path / ("in.fq.gz" if case["gzip"] else "in.txt")
+    data = case["text"].encode("latin-1")
+    path.write_bytes(gzip.compress(data) if case["gzip"] else data)
+    print(f"cli reader {case['case']}", flush=True)
+    p = subprocess.run([CLI, str(path), "-r", "1"], capture_output=True, timeout=120)
+    assert p.returncode == case["returncode"], p.stderr
+    assert p.stdout.decode("latin-1") == case["stdout"]
