@@ -230,6 +230,29 @@ def estimate_form(ctx, params, batch, steps: int, log) -> dict:
             "reruns": int(st.reruns), "steps": steps, "texts": texts}
 
 
+def wide_form(ctx, params, batch, pred, steps: int, log) -> dict:
+    """The same batch and schedule with every DP call's matrices stored as int32
+    (MPOA_FORM_WIDE, minipoa's width) instead of int16 where the values fit: windows/s, for
+    the storage A/B."""
+    from poa_amd import api
+    b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
+    if pred is not None:
+        b.set_predictions(pred)
+    b.set_forms(api.FORM_WIDE)
+    b.execute()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.execute()
+    wall = time.perf_counter() - t0
+    st = b.stats()
+    texts = b.results().texts
+    b.free()
+    wps = batch.n_windows * steps / wall
+    log(f"int32 storage: {wps:.1f} windows/s, {st.narrow_cells} int16 cells")
+    return {"windows_per_s": round(wps, 1), "reruns": int(st.reruns), "steps": steps,
+            "texts": texts}
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on this workload, from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration);
@@ -318,6 +341,7 @@ def main() -> int:
     tp = time.perf_counter()
     b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
     learned = None
+    pred = None
     if not args.no_learned and os.path.exists(sched.MODEL):
         # Learned scheduling: per-call features -> network on the GPU -> per-window cost,
         # arena sizes and tile choice. Host feature extraction and inference are part of the
@@ -386,7 +410,10 @@ def main() -> int:
     lst = {ls.launch: ls for ls in b.launch_stats()}
     dom = max(lst.values(), key=lambda ls: ls.cells)
     dom_ebar = dom.edge_cells / max(dom.cells, 1)
-    bytes_per_cell = 4 * (3 + 2 * dom_ebar)
+    # Score bytes per cell of the launch: 2 for the calls stored as int16, 4 for the rest.
+    dom_narrow = sum(w.narrow_cells for w in ws if w.launch == dom.launch)
+    score_bytes = 4 - 2 * dom_narrow / max(dom.cells, 1)
+    bytes_per_cell = score_bytes * (3 + 2 * dom_ebar)
     alg_bytes = dom.cells * bytes_per_cell
     dom_ms = float(np.mean(launch_ms[dom.launch]))
     achieved = alg_bytes / (dom_ms / 1000.0) / 1e9
@@ -421,6 +448,10 @@ def main() -> int:
             "band_cells": int(st.cells),
             "edge_cells_per_cell": round(ebar, 4),
             "logical_cells": int(st.logical_cells),
+            "score_storage": {"int16_cells": int(st.narrow_cells),
+                              "int32_cells": int(st.cells - st.narrow_cells),
+                              "note": "int32 arithmetic; H, D and I stored as int16 for the "
+                                      "DP calls whose values fit, else int32"},
             "reruns": int(st.reruns),
             "launches": int(st.launches),
             "slots": int(st.slots),
@@ -442,6 +473,7 @@ def main() -> int:
             "windows": int(dom.windows),
             "band_cells": int(dom.cells),
             "bytes_per_cell": round(bytes_per_cell, 3),
+            "score_bytes": round(score_bytes, 4),
             "alg_bytes": int(alg_bytes),
             "all_launches_ms": round(float(np.mean(kernel_ms)), 3),
         },
@@ -472,6 +504,13 @@ def main() -> int:
                         "reruns": int(st.reruns)},
             "read_length_estimate": {k: v for k, v in est.items() if k != "texts"},
             "outputs_identical": est["texts"] == res.texts}
+    if not args.no_ab:
+        wide = wide_form(ctx, params, batch, pred, 2, log)
+        line["storage_ab"] = {
+            "int16_where_fits": {"windows_per_s": round(windows_per_s, 1),
+                                 "int16_cells": int(st.narrow_cells)},
+            "int32_only": {k: v for k, v in wide.items() if k != "texts"},
+            "outputs_identical": wide["texts"] == res.texts}
     if not args.no_ingest:
         line["ingest"] = ingest(batch, cores, log)
     if not args.no_single:

@@ -4,8 +4,10 @@
  * interface it replaces. Plain pointers and sizes only; the library owns its device memory
  * through an mpoa_ctx, callers own every host buffer they pass.
  *
- * Scores are minipoa's int32 arithmetic (src/align.cpp:21-22); results are bit-identical to
- * minipoa on the same inputs and flags.
+ * Scores are minipoa's int32 arithmetic (src/align.cpp:21-22); a batched DP call whose values
+ * fit int16 stores its H, D and I matrices as int16 (checked on the device; a call that does not
+ * fit is re-run with int32 storage). Results are bit-identical to minipoa on the same inputs and
+ * flags.
  */
 #ifndef MPOA_H
 #define MPOA_H
@@ -267,6 +269,8 @@ typedef struct mpoa_batch_stats {
   int32_t graph_builds;   /* graphs built over the batch's life (a rebuild follows any change
                              of a launch's arguments) */
   int64_t logical_cells;  /* sum over all DP rows of Me - Ms + 1 (cells counts 8*block_num) */
+  int64_t narrow_cells;   /* cells of the DP calls whose matrices were stored as int16 (the rest
+                             as int32, minipoa's width; both give the same results) */
 } mpoa_batch_stats;
 
 /* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window
@@ -305,6 +309,7 @@ typedef struct mpoa_window_stat {
   int64_t ticks[8];
   int64_t t_begin, t_end;                /* device wall clock at the window's start and end */
   int64_t logical_cells;
+  int64_t narrow_cells;                  /* cells of its DP calls stored as int16 */
 } mpoa_window_stat;
 int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, double* tick_hz);
 
@@ -329,12 +334,13 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
  * past 512 nodes); MPOA_FORM_BT_LANE, the one-lane backtrack (a row with more than 63
  * predecessors); MPOA_FORM_ADD_LANE, the one-lane add_path (a repeated edge);
  * MPOA_FORM_CONS_ROUNDS, the -S consensus settling every block in rounds (a rank with more than
- * eight predecessors in its block). 0 (the default) restores the normal choice. Results do not
- * change; only speed does. mpoa_set_forms sets the forms of the batches the context creates
- * afterwards, mpoa_align_reads' included. */
+ * eight predecessors in its block); MPOA_FORM_WIDE, int32 storage of the DP matrices for every
+ * call (a call whose values do not fit int16: minipoa's own width). 0 (the default) restores
+ * the normal choice. Results do not change; only speed does. mpoa_set_forms sets the forms of
+ * the batches the context creates afterwards, mpoa_align_reads' included. */
 enum {
   MPOA_FORM_SORT_HBM = 1, MPOA_FORM_SORT_LANE = 2, MPOA_FORM_BT_LANE = 4, MPOA_FORM_ADD_LANE = 8,
-  MPOA_FORM_CONS_ROUNDS = 16
+  MPOA_FORM_CONS_ROUNDS = 16, MPOA_FORM_WIDE = 32
 };
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms);
 int mpoa_set_forms(mpoa_ctx* ctx, int32_t forms);

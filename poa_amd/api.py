@@ -20,6 +20,7 @@ from ._lib import PRODUCT_SO
 MPOA_OK = 0
 MPOA_E_OUT_CAP = -4
 FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE, FORM_CONS_ROUNDS = 1, 2, 4, 8, 16
+FORM_WIDE = 32  # int32 storage of the DP matrices for every call (no int16 form)
 ERRORS = {
     -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
     -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW", -9: "E_SCRATCH",
@@ -81,7 +82,7 @@ class WindowStat(ctypes.Structure):
                 ("launch", ctypes.c_int32), ("cells", ctypes.c_int64),
                 ("edge_cells", ctypes.c_int64), ("ticks", ctypes.c_int64 * 8),
                 ("t_begin", ctypes.c_int64), ("t_end", ctypes.c_int64),
-                ("logical_cells", ctypes.c_int64)]
+                ("logical_cells", ctypes.c_int64), ("narrow_cells", ctypes.c_int64)]
 
 
 class LaunchStat(ctypes.Structure):
@@ -97,7 +98,8 @@ class BatchStats(ctypes.Structure):
                 ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("pool_bytes", ctypes.c_int64),
                 ("graph_launches", ctypes.c_int32), ("graph_builds", ctypes.c_int32),
-                ("logical_cells", ctypes.c_int64)]
+                ("logical_cells", ctypes.c_int64),
+                ("narrow_cells", ctypes.c_int64)]
 
 
 _lib = None

@@ -209,10 +209,11 @@ int upload_order(Bin& bin, hipStream_t stream) {
   return MPOA_OK;
 }
 
-// The persistent kernel's arguments for a provisioned bin.
+// The persistent kernel's arguments for a provisioned bin. Re-runs store the DP matrices as
+// int32: a window re-run because a call's values left int16 must not take that form again.
 KernelArgs args_of(const mpoa_batch* b, const Bin& bin) {
   KernelArgs a{};
-  a.forms = b->forms;
+  a.forms = b->forms | (&bin == &b->rerun ? MPOA_FORM_WIDE : 0);
   a.scores = b->sc;
   a.caps = bin.caps;
   a.read_seq = b->d_seq.p;
@@ -541,7 +542,7 @@ int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vecto
                      : b->bin_of[w] == 3 ? b->wide
                                          : b->small;
     KernelArgs a{};
-    a.forms = b->forms;
+    a.forms = b->forms | (worst ? MPOA_FORM_WIDE : 0);
     a.scores = b->sc;
     a.caps = bin.caps;
     a.read_seq = b->d_seq.p;
@@ -1199,6 +1200,7 @@ int mpoa_batch_results(mpoa_batch* b, char* out, int64_t out_cap, int64_t* out_o
       stats->cells += b->res[w].cells;
       stats->edge_cells += b->res[w].edge_cells;
       stats->logical_cells += b->res[w].logical;
+      stats->narrow_cells += b->res[w].narrow_cells;
       stats->dp_calls += std::max(0, b->info[w].n_reads - 1);
     }
     stats->reruns = b->reruns;
@@ -1274,6 +1276,7 @@ int mpoa_batch_window_stats(mpoa_batch* b, mpoa_window_stat* out, int32_t cap, d
     o.t_end = r.t_end;
     o.cells = r.cells;
     o.logical_cells = r.logical;
+    o.narrow_cells = r.narrow_cells;
     o.edge_cells = r.edge_cells;
     for (int k = 0; k < 8; k++) o.ticks[k] = r.ticks[k];
   }
@@ -1293,7 +1296,7 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode) {
 }
 
 constexpr int kAllForms = MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE | MPOA_FORM_BT_LANE |
-                          MPOA_FORM_ADD_LANE | MPOA_FORM_CONS_ROUNDS;
+                          MPOA_FORM_ADD_LANE | MPOA_FORM_CONS_ROUNDS | MPOA_FORM_WIDE;
 
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms) {
   if (!b || (forms & ~kAllForms) != 0) return MPOA_E_ARG;
@@ -1588,7 +1591,8 @@ int mpoa_align_reads_noalloc(mpoa_ctx* ctx, const mpoa_params* params, int32_t n
   }
   uint8_t* base = ctx->scratch;
   KernelArgs a{};
-  a.forms = ctx->forms;
+  // int32 storage only: this entry point has no re-run for a call whose values leave int16
+  a.forms = ctx->forms | MPOA_FORM_WIDE;
   a.scores = sc;
   a.caps = caps;
   a.read_seq = reinterpret_cast<const uint8_t*>(d_seq);

@@ -131,6 +131,7 @@ struct WinResult {
   int nodes, edges;
   int64_t ticks[PH_N];
   int64_t t_begin, t_end;  // device wall clock at the window's start and end
+  int64_t narrow_cells;    // band cells of the calls whose H, D and I were stored as int16
 };
 
 // One DP call (one read against the graph of the reads before it).

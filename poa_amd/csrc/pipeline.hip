@@ -67,6 +67,9 @@ struct WinShared {
   int scan[16];
   int row_bs, row_be;
   int row_wmax;   // widest row of the current static-band plan (8*block_num)
+  uint32_t fmin;  // int16 form: smallest finite H of the call, less FINITE_CUT
+  bool narrow_bad;       // the call's int16 form did not hold its values: re-run in int32
+  int64_t narrow_cells;  // band cells of the window's calls in the int16 form
   bool q_lds;
   int q_cap;
   int64_t ticks[PH_N];
@@ -833,6 +836,66 @@ __device__ __forceinline__ int block_excl_max(int v, WinShared& S, int* total, i
   return max(excl, before);
 }
 
+// Score storage. A DP call keeps H, D and I either as int32 (minipoa's own width) or, when
+// every finite value of the call fits, as int16 ("narrow"): finite values are stored as they
+// are and every value derived from NEG_INF is stored as -32768, which reads back as NEG_INF.
+// In registers the arithmetic stays int32, so a finite cell gets exactly minipoa's value and a
+// cell derived from NEG_INF stays far below every finite one (minipoa's NEG_INF offsets differ,
+// which no comparison between a finite and a NEG_INF-derived value can see).
+constexpr int16_t N16 = -32768;        // stored form of every NEG_INF-derived value
+constexpr int FINITE_CUT = -(1 << 29);  // values above this are finite (NEG_INF offsets stay below)
+__device__ __forceinline__ int widen(int v) { return v; }
+__device__ __forceinline__ int widen(int16_t v) { return v == N16 ? NEG_INF : (int)v; }
+__device__ __forceinline__ int widen_lo(uint32_t w) { return widen((int16_t)(w & 0xffff)); }
+__device__ __forceinline__ int widen_hi(uint32_t w) { return widen((int16_t)(w >> 16)); }
+// A value stored as int16: NEG_INF-derived values clamp to N16; finite values are in range
+// (the call's bound check).
+__device__ __forceinline__ uint32_t pack16(int a, int b) {
+  return ((uint32_t)max(a, -32768) & 0xffffu) | ((uint32_t)max(b, -32768) << 16);
+}
+template <class T> __device__ __forceinline__ T store_of(int v);
+template <> __device__ __forceinline__ int store_of<int>(int v) { return v; }
+template <> __device__ __forceinline__ int16_t store_of<int16_t>(int v) {
+  return (int16_t)max(v, -32768);
+}
+
+// CPL consecutive stored scores at base[idx] (idx a multiple of CPL), widened to int32: one
+// aligned vector load.
+template <int CPL>
+__device__ __forceinline__ void load_cols(const int16_t* base, int idx, int* out) {
+  if constexpr (CPL == 1) {
+    out[0] = widen(base[idx]);
+  } else if constexpr (CPL == 2) {
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(base + idx);
+    out[0] = widen_lo(v); out[1] = widen_hi(v);
+  } else if constexpr (CPL == 4) {
+    const uint2 v = *reinterpret_cast<const uint2*>(base + idx);
+    out[0] = widen_lo(v.x); out[1] = widen_hi(v.x); out[2] = widen_lo(v.y); out[3] = widen_hi(v.y);
+  } else {
+    static_assert(CPL == 8, "CPL");
+    const uint4 v = *reinterpret_cast<const uint4*>(base + idx);
+    out[0] = widen_lo(v.x); out[1] = widen_hi(v.x); out[2] = widen_lo(v.y); out[3] = widen_hi(v.y);
+    out[4] = widen_lo(v.z); out[5] = widen_hi(v.z); out[6] = widen_lo(v.w); out[7] = widen_hi(v.w);
+  }
+}
+template <int CPL>
+__device__ __forceinline__ void load_cols(int16_t* base, int idx, int* out) {
+  load_cols<CPL>(const_cast<const int16_t*>(base), idx, out);
+}
+template <int CPL>
+__device__ __forceinline__ void store_cols(int16_t* base, int idx, const int* v) {
+  if constexpr (CPL == 1) {
+    base[idx] = store_of<int16_t>(v[0]);
+  } else if constexpr (CPL == 2) {
+    *reinterpret_cast<uint32_t*>(base + idx) = pack16(v[0], v[1]);
+  } else if constexpr (CPL == 4) {
+    *reinterpret_cast<uint2*>(base + idx) = make_uint2(pack16(v[0], v[1]), pack16(v[2], v[3]));
+  } else {
+    *reinterpret_cast<uint4*>(base + idx) = make_uint4(pack16(v[0], v[1]), pack16(v[2], v[3]),
+                                                       pack16(v[4], v[5]), pack16(v[6], v[7]));
+  }
+}
+
 // CPL consecutive int32 at base[idx] (idx a multiple of CPL): one aligned vector load.
 template <int CPL, class P>
 __device__ __forceinline__ void load_cols(P base, int idx, int* out) {
@@ -882,7 +945,7 @@ __device__ __forceinline__ void accumulate(P Hrow, P Drow, int len, int pofs, in
   int hv[CPL], dv[CPL];
   load_cols<CPL>(Hrow, safe, hv);
   load_cols<CPL>(Drow, safe, dv);
-  const int hprev = Hrow[min(max(idx - 1, 0), len - 1)];
+  const int hprev = widen(Hrow[min(max(idx - 1, 0), len - 1)]);
   // A predecessor whose bands cover the row's whole pass (the common case, a wave-uniform
   // test) needs no per-column masks.
   const bool whole = mlo <= c0 && mhi >= c1 && dlo <= c0 && dhi >= c1;
@@ -943,10 +1006,19 @@ __device__ __forceinline__ int cache_slot(int meta_row, int p) {
   return hit ? (int)__builtin_ctzll(hit) : -1;
 }
 
-template <int CPL, class RingT>
-__device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpScores& sc, int p,
-                                             int pbase, int cb, int c0, int c1, const int* qc,
-                                             int* h, int* d, int meta_row, int meta_bsbe, int sw) {
+// The DP matrices of one call in their storage type T (int or int16_t): HBM rows (H, D, I at
+// row_off) and the LDS row cache (slot k at RH / RD + k * sw).
+template <class T>
+struct DpMats {
+  T *H, *D, *I;
+  T *RH, *RD;
+};
+
+template <int CPL, class T>
+__device__ __forceinline__ void pred_contrib(const Arena& A, const DpMats<T>& X, const DpScores& sc,
+                                             int p, int pbase, int cb, int c0, int c1,
+                                             const int* qc, int* h, int* d, int meta_row,
+                                             int meta_bsbe, int sw) {
   if (p == 0) pbase = CODE_N;
   const int slot = cache_slot(meta_row, p);
   if (slot >= 0) {
@@ -954,13 +1026,13 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
     const int bs = bsbe & 0xffff, be = bsbe >> 16;
     const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
     const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
-    accumulate<CPL>(R.H + slot * sw, R.D + slot * sw, sw, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
-                    pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
+    accumulate<CPL>(X.RH + slot * sw, X.RD + slot * sw, sw, bs * Q, cb, c0, c1, mlo, mhi, dlo,
+                    dhi, pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
   } else {
     const int bs = A.Bs[p], be = A.Be[p], off = A.row_off[p];
     const int mlo = max(c0, bs == 0 ? 0 : (bs + 1) * Q), mhi = min(c1, (be - 1) * Q);
     const int dlo = max(c0, bs * Q), dhi = min(c1, (be - 1) * Q);
-    accumulate<CPL>(A.H + off, A.D + off, (be - bs) * Q, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
+    accumulate<CPL>(X.H + off, X.D + off, (be - bs) * Q, bs * Q, cb, c0, c1, mlo, mhi, dlo, dhi,
                     pbase, qc, sc.match, sc.mismatch, sc.o1, sc.e1, h, d);
     drain_vmem();
   }
@@ -977,19 +1049,22 @@ __device__ __forceinline__ void pred_contrib(const Arena& A, RingT& R, const DpS
 // the cache, predecessors after the third, a query too long for LDS) sit in branches of their
 // own.
 // Returns the first argmax column (local) of the row's H, for the adaptive band; *np_in gets
-// the number of in-window predecessors.
-template <int NT, int CPL, bool QLDS, class RingT>
-__device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpScores& sc, int i,
-                      const RowRec& rr, int bs, int be, bool want_argmax, int* np_in,
-                      int meta_row, int meta_bsbe, int sw, int slot_i, int& scan_ph) {
+// the number of in-window predecessors. In the int16 form, fmin keeps the thread's smallest
+// finite H as an unsigned offset from FINITE_CUT (NEG_INF-derived values wrap to above 2^31
+// and never lower it).
+template <int NT, int CPL, bool QLDS, class T, class RingT>
+__device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, const DpMats<T>& X,
+                                      const DpScores& sc, int i, const RowRec& rr, int bs, int be,
+                                      bool want_argmax, int* np_in, int meta_row, int meta_bsbe,
+                                      int sw, int slot_i, int& scan_ph, uint32_t& fmin) {
   const int tid = threadIdx.x;
   const int c0 = bs * Q, W = (be - bs - 1) * Q, c1 = c0 + W;
-  int* Hi = A.H + rr.row_off;  // column c at Hi[c - c0]
-  int* Di = A.D + rr.row_off;
-  int* Ii = A.I + rr.row_off;
+  T* Hi = X.H + rr.row_off;  // column c at Hi[c - c0]
+  T* Di = X.D + rr.row_off;
+  T* Ii = X.I + rr.row_off;
   const bool cache_i = slot_i >= 0;
-  int* RHi = R.H + (cache_i ? slot_i : 0) * sw;  // column c at RHi[c - c0]
-  int* RDi = R.D + (cache_i ? slot_i : 0) * sw;
+  T* RHi = X.RH + (cache_i ? slot_i : 0) * sw;  // column c at RHi[c - c0]
+  T* RDi = X.RD + (cache_i ? slot_i : 0) * sw;
   const int o = sc.o1, e = sc.e1;
   const int beg = S.beg_i;
   const int n3 = min(rr.np, 3);
@@ -1010,7 +1085,7 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
       const int p = prank - beg;
       if (p < 0) continue;
       npi++;
-      pred_contrib<CPL>(A, R, sc, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d, meta_row,
+      pred_contrib<CPL>(A, X, sc, p, (rr.pk >> (3 * k)) & 7, cb, c0, c1, qc, h, d, meta_row,
                         meta_bsbe, sw);
     }
     if (rr.np > 3) {
@@ -1020,7 +1095,7 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
         drain_vmem();
         if (p < 0) continue;
         npi++;
-        pred_contrib<CPL>(A, R, sc, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe, sw);
+        pred_contrib<CPL>(A, X, sc, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe, sw);
       }
       drain_vmem();
     }
@@ -1045,6 +1120,7 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
         hv[l] = max(g[l], iv[l]);
         run = max(run, t[l]);
         if (want_argmax && hv[l] > best) { best = hv[l]; best_j = j; }
+        if constexpr (sizeof(T) == 2) fmin = min(fmin, (uint32_t)(hv[l] - FINITE_CUT));
       }
       const int j0 = cb - c0;
       store_cols<CPL>(Hi, j0, hv);
@@ -1059,7 +1135,7 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
   }
   *np_in = npi;
   // NEG_INF sentinel block after the band (align.cpp:1505-1506,1515).
-  if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = NEG_INF;
+  if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = store_of<T>(NEG_INF);
   if (!want_argmax) return 0;
   // First argmax over the row (value desc, column asc).
 #pragma unroll
@@ -1150,12 +1226,14 @@ __device__ __forceinline__ bool plan_static(const Arena& A, WinShared& S, const 
 
 // Backtrack (src/align.cpp:1572-1736), one lane. Emits res[acj-1] for every consumed query
 // column, so the result is already in forward order.
-__device__ __forceinline__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc) {
+template <class T>
+__device__ __forceinline__ int backtrack(const Arena& A, WinShared& S, const DpScores& sc,
+                                         const DpMats<T>& X) {
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
-  auto Hat = [&](int r, int j) { return A.H[A.row_off[r] + j]; };
-  auto Dat = [&](int r, int j) { return A.D[A.row_off[r] + j]; };
-  auto Iat = [&](int r, int j) { return A.I[A.row_off[r] + j]; };
+  auto Hat = [&](int r, int j) { return widen(X.H[A.row_off[r] + j]); };
+  auto Dat = [&](int r, int j) { return widen(X.D[A.row_off[r] + j]); };
+  auto Iat = [&](int r, int j) { return widen(X.I[A.row_off[r] + j]); };
   int i = n - 1, acj = m, op = ALL_OP;
   auto emit_sub = [&](int pid, int p) {
     const int pbase = p == 0 ? CODE_N : A.base[pid];
@@ -1300,8 +1378,9 @@ __device__ __forceinline__ int bt_pick(bool cand, int w, int lane) {
 // Lanes 0..K (K <= 63) take part. Moves of the leading lanes that pass are taken (at most K
 // per call, two round trips), and logged as their predecessor's local rank. Returns their number; *nbs, *nbe, *noff, *nrr get
 // the geometry and record of the row reached.
-__device__ __forceinline__ int bt_run(const Arena& A, const WinShared& S, int match, int i,
-                                      int acj, int op, int K, int* log, int* nbs, int* nbe,
+template <class T>
+__device__ __forceinline__ int bt_run(const Arena& A, const WinShared& S, const T* H, int match,
+                                      int i, int acj, int op, int K, int* log, int* nbs, int* nbe,
                                       int* noff, RowRec* nrr) {
   const int lane = threadIdx.x & 63;
   const int beg = S.beg_i;
@@ -1317,8 +1396,9 @@ __device__ __forceinline__ int bt_run(const Arena& A, const WinShared& S, int ma
     if (a > 0) qc = A.qseq[a - 1];
   }
   const bool in_band = live && a >= bs * Q && a <= (be - 1) * Q;
-  int h = 0;
-  if (in_band) h = A.H[off + a - bs * Q];
+  T rh = 0;  // widened after the branch: no wait inside it
+  if (in_band) rh = H[off + a - bs * Q];
+  const int h = widen(rh);
   const int up_bs = __shfl_down(bs, 1, 64), up_be = __shfl_down(be, 1, 64);
   const int up_h = __shfl_down(h, 1, 64);
   bool ok = in_band && r >= 1 && a >= 1 && lane < K && rec.np >= 1 && rec.np <= 3;
@@ -1365,8 +1445,9 @@ __device__ __forceinline__ int bt_run(const Arena& A, const WinShared& S, int ma
   return L;
 }
 
+template <class T>
 __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc,
-                                              int* log) {
+                                              const DpMats<T>& X, int* log) {
   const int lane = threadIdx.x & 63;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
@@ -1388,7 +1469,7 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
     if (run && (op & M_OP) && acj > 0 && i > 0) {
       int nbs, nbe, noff;
       RowRec nrr;
-      const int L = bt_run(A, S, match, i, acj, op, width, log, &nbs, &nbe, &noff, &nrr);
+      const int L = bt_run(A, S, X.H, match, i, acj, op, width, log, &nbs, &nbe, &noff, &nrr);
       run = L == width;  // a full run may go on; otherwise the next move is not such a match
       width = run ? min(2 * width, 63) : 8;
       if (!run) streak = 0;
@@ -1438,28 +1519,32 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
     const int pj = acj - pbs * Q;
     const bool m_in = valid && acj > 0 && pj - 1 >= 0 && pj - 1 < (pbe - pbs - 1) * Q;
     const bool d_in = valid && pbs <= acj / Q && acj / Q < pbe - 1;
-    int hdiag = 0, hup = 0, dup = 0;
-    if (m_in) hdiag = A.H[poff + pj - 1];
+    // The stored values are loaded in the branches and widened after them, so that every
+    // load of the move is issued before the one wait for them.
+    T r_hdiag = 0, r_hup = 0, r_dup = 0;
+    if (m_in) r_hdiag = X.H[poff + pj - 1];
     if (d_in) {
-      hup = A.H[poff + pj];
-      dup = A.D[poff + pj];
+      r_hup = X.H[poff + pj];
+      r_dup = X.D[poff + pj];
     }
-    int c_h = 0, c_d = 0, c_i = 0, c_hm = 0, c_im = 0, c_q = 0;
+    T r_h = 0, r_d = 0, r_i = 0, r_hm = 0, r_im = 0;
+    int c_q = 0;
     if (lane == 63) {
-      c_h = A.H[off_i + j];
-      c_d = A.D[off_i + j];
-      c_i = A.I[off_i + j];
+      r_h = X.H[off_i + j];
+      r_d = X.D[off_i + j];
+      r_i = X.I[off_i + j];
       if (j >= 1) {
-        c_hm = A.H[off_i + j - 1];
-        c_im = A.I[off_i + j - 1];
+        r_hm = X.H[off_i + j - 1];
+        r_im = X.I[off_i + j - 1];
       }
       if (acj > 0) c_q = A.qseq[acj - 1];
     }
-    const int h = __builtin_amdgcn_readlane(c_h, 63);
-    const int dv = __builtin_amdgcn_readlane(c_d, 63);
-    const int iv = __builtin_amdgcn_readlane(c_i, 63);
-    const int hm = __builtin_amdgcn_readlane(c_hm, 63);
-    const int im = __builtin_amdgcn_readlane(c_im, 63);
+    const int hdiag = widen(r_hdiag), hup = widen(r_hup), dup = widen(r_dup);
+    const int h = __builtin_amdgcn_readlane(widen(r_h), 63);
+    const int dv = __builtin_amdgcn_readlane(widen(r_d), 63);
+    const int iv = __builtin_amdgcn_readlane(widen(r_i), 63);
+    const int hm = __builtin_amdgcn_readlane(widen(r_hm), 63);
+    const int im = __builtin_amdgcn_readlane(widen(r_im), 63);
     const int qc = __builtin_amdgcn_readlane(c_q, 63);
     // kind: 1 substitution move (logged), 2 deletion move, 3 insertion (logged)
     int k = -1, kind = 0, nop = op;
@@ -1643,10 +1728,14 @@ struct RingShape {
   static constexpr int QCAP = NT == 64 ? 1536 : NT <= 256 ? 4096 : 24576;
 };
 
-// One poa() call on the device graph: rows [beg_i, end_i] of the current rank order.
-template <int NT, int CPL, bool QLDS, class RingT>
+template <int NT, int CPL, bool QLDS, class T, class RingT>
+__device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShared& S, RingT& R,
+                                         bool adaptive, int w, int full_len, int64_t t0);
+
+// One poa() call on the device graph: rows [beg_i, end_i] of the current rank order. NARROW:
+// the int16 form may be taken (batched windows; the other kernels keep int32 only).
+template <int NT, int CPL, bool QLDS, bool NARROW, class RingT>
 __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShared& S, RingT& R, bool adaptive) {
-  constexpr int RING = RingT::kRing, WR = RingT::kWR;
   const DpScores& sc = a.scores;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int tid = threadIdx.x;
@@ -1680,6 +1769,41 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     }
     __syncthreads();
   }
+  // The int16 form when the call's static bound allows it: every path score is at most
+  // match per query column consumed (gap scores are not positive, a mismatch scores at most a
+  // match), so H, D and I stay at or below match * m; the lower side is checked on the
+  // finite H values as they are computed (a finite D or I is at least a finite H plus the
+  // gap-open score). Adaptive calls stay int32: a row without a finite cell picks its argmax
+  // among minipoa's NEG_INF offsets, which the int16 form does not keep.
+  if constexpr (NARROW && QLDS) {
+    const bool narrow = !adaptive && !(a.forms & MPOA_FORM_WIDE) && sc.o1 <= 0 && sc.e1 <= 0 &&
+                        sc.o1 >= -1024 && sc.match >= 0 && sc.mismatch <= sc.match &&
+                        (int64_t)sc.match * m <= 32767;
+    if (narrow) {
+      poa_rows<NT, CPL, QLDS, int16_t>(a, A, S, R, adaptive, w, full_len, t0);
+      return;
+    }
+  }
+  poa_rows<NT, CPL, QLDS, int>(a, A, S, R, adaptive, w, full_len, t0);
+}
+
+// The DP rows, the score and the backtrack of one poa() call whose plan is filled, with
+// H, D and I stored as T.
+template <int NT, int CPL, bool QLDS, class T, class RingT>
+__device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShared& S, RingT& R,
+                                         bool adaptive, int w, int full_len, int64_t t0) {
+  constexpr int RING = RingT::kRing, WR = RingT::kWR;
+  const DpScores& sc = a.scores;
+  const int n = S.n_rows, m = S.m, beg = S.beg_i;
+  const int tid = threadIdx.x;
+  DpMats<T> X;
+  X.H = reinterpret_cast<T*>(A.H);
+  X.D = reinterpret_cast<T*>(A.D);
+  X.I = reinterpret_cast<T*>(A.I);
+  X.RH = reinterpret_cast<T*>(R.H);
+  X.RD = reinterpret_cast<T*>(R.D);
+  // columns of each of the ring's H and D arrays in the storage type
+  constexpr int RCOLS = RING * WR * (int)(sizeof(int) / sizeof(T));
   // LDS row cache: slots of sw columns (H and D each), as many as fit the LDS arrays (at most
   // 64, one per lane of the directory), sw the widest row of the call (static band) or a
   // typical adaptive band. A row takes a slot when a later row reads it and it fits; it keeps
@@ -1695,26 +1819,29 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     sw = min(full_len - Q, (2 * w + 128 + Q - 1) / Q * Q);
   }
   sw = max(sw, Q);
-  const int ns = min(64, (RING * WR) / sw);
+  const int ns = min(64, RCOLS / sw);
   int meta_row = -1, meta_bsbe = 0, meta_last = -1;
   int scan_ph = 0;  // half of S.scan the next row scan writes
+  uint32_t fmin = 0xffffffffu;  // int16 form: smallest finite H of this thread, less FINITE_CUT
   // Row 0 (align.cpp:1415-1425): M, D = NEG_INF over blocks [Bs0, Be0) from the row start;
   // M[0][0] = 0. I of row 0 is never read; it is set to NEG_INF. Bs0 is always 0.
   {
     const int be0 = A.Be[0];
     const int len0 = be0 * Q, w0 = (be0 - 1) * Q;
     const int last0 = A.rowrec[12LL * beg + 11];
-    int* H0 = A.H + A.row_off[0];
-    int* D0 = A.D + A.row_off[0];
-    int* I0 = A.I + A.row_off[0];
+    T* H0 = X.H + A.row_off[0];
+    T* D0 = X.D + A.row_off[0];
+    T* I0 = X.I + A.row_off[0];
     const bool cache0 = ns > 0 && w0 <= sw && last0 > beg;
+    const T zero = store_of<T>(0), ninf = store_of<T>(NEG_INF);
+    if (tid == 0) S.fmin = 0xffffffffu;
     for (int j = tid; j < len0; j += NT) {
-      H0[j] = j == 0 ? 0 : NEG_INF;
-      D0[j] = NEG_INF;
-      I0[j] = NEG_INF;
+      H0[j] = j == 0 ? zero : ninf;
+      D0[j] = ninf;
+      I0[j] = ninf;
       if (cache0 && j < w0) {
-        R.H[j] = j == 0 ? 0 : NEG_INF;
-        R.D[j] = NEG_INF;
+        X.RH[j] = j == 0 ? zero : ninf;
+        X.RD[j] = ninf;
       }
     }
     if (cache0 && lane == 0) {
@@ -1804,8 +1931,8 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
       }
     }
     int np_in = 0;
-    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, sc, i, rr, bs, be, adaptive, &np_in, meta_row,
-                                         meta_bsbe, sw, slot_i, scan_ph);
+    const int mj = dp_row<NT, CPL, QLDS>(A, S, R, X, sc, i, rr, bs, be, adaptive, &np_in,
+                                         meta_row, meta_bsbe, sw, slot_i, scan_ph, fmin);
     if (slot_i >= 0 && lane == slot_i) {
       meta_row = i;
       meta_bsbe = bs | (be << 16);
@@ -1820,6 +1947,11 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
       lds_barrier<NT>();
     }
   }
+  if constexpr (sizeof(T) == 2) {
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) fmin = min(fmin, (uint32_t)__shfl_xor((int)fmin, s, 64));
+    if (lane == 0) atomicMin(&S.fmin, fmin);
+  }
   __syncthreads();
   const int64_t t2 = now();
   if (tid == 0) {
@@ -1830,18 +1962,31 @@ __device__ __forceinline__ void poa_call(const KernelArgs& a, Arena& A, WinShare
     S.call_ecells = ecells;
     S.logical += S.call_logical;
     S.call_width = wmax;
-    S.score = A.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)];
+    S.score = widen(X.H[A.row_off[n - 1] + (m - A.Bs[n - 1] * Q)]);
+    S.narrow_bad = false;
+    if constexpr (sizeof(T) == 2) {
+      // Every finite H at least -32767 - o1 keeps every finite D and I at -32767 or above,
+      // so no finite value was stored as N16; a NEG_INF-derived end cell would send the
+      // backtrack through minipoa's NEG_INF offsets. Either failing: the window is re-run
+      // in int32.
+      const int64_t lo = (int64_t)S.fmin + FINITE_CUT;
+      S.narrow_bad = S.fmin == 0xffffffffu || lo < -32767LL - sc.o1 || S.score <= FINITE_CUT;
+      if (!S.narrow_bad) S.narrow_cells += cells;
+    }
   }
+  __syncthreads();
   // The backtrack's move log goes to LDS (the ring's rows are dead now) when the query fits: a
   // global store there would make the walk's next loads wait for it as well.
   int* log = m <= 2 * RING * WR ? reinterpret_cast<int*>(&R) : A.res_from;  // H, D first
   if (__builtin_amdgcn_readfirstlane(tid >> 6) == 0) {  // wave 0 (a wave-uniform branch)
-    const int rc = (a.forms & MPOA_FORM_BT_LANE) ? 1 : backtrack_wave(A, S, sc, log);
+    const int rc = S.narrow_bad ? MPOA_E_OVERFLOW
+                   : (a.forms & MPOA_FORM_BT_LANE) ? 1
+                                                   : backtrack_wave(A, S, sc, X, log);
     if (tid == 0) {
       // A row with more than 63 predecessors: the one-lane backtrack, which writes the
       // res_t entries itself.
       S.bt_serial = rc == 1;
-      S.bt_rc = rc == 1 ? backtrack(A, S, sc) : rc;
+      S.bt_rc = rc == 1 ? backtrack(A, S, sc, X) : rc;
     }
   }
   __syncthreads();
@@ -2283,6 +2428,7 @@ __device__ __forceinline__ void window_outputs(const KernelArgs& a, Arena& A, Wi
     res.cells = S.cells;
     res.edge_cells = S.edge_cells;
     res.logical = S.logical;
+    res.narrow_cells = S.narrow_cells;
     res.nodes = S.n_nodes;
     res.edges = S.n_in_e;
   }
@@ -2295,7 +2441,7 @@ __device__ __forceinline__ void window_begin(const KernelArgs& a, Arena& A, WinS
     a.result[w].t_begin = now();
     S.status = MPOA_OK;
     S.n_nodes = 0; S.n_in_e = 0; S.n_out_e = 0;
-    S.cells = 0; S.edge_cells = 0; S.logical = 0;
+    S.cells = 0; S.edge_cells = 0; S.logical = 0; S.narrow_cells = 0;
     S.cons_len = 0;
     for (int k = 0; k < PH_N; k++) S.ticks[k] = 0;
     new_node(A, S, a.caps, CODE_N);  // source 0 (graph.cpp:129-135)
@@ -2333,8 +2479,8 @@ __device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinSha
       }
       __syncthreads();
       const bool adaptive = a.scores.ab_band != 0 && a.scores.f > 0;
-      if (S.q_lds) poa_call<NT, CPL, true>(a, A, S, R, adaptive);
-      else poa_call<NT, CPL, false>(a, A, S, R, adaptive);
+      if (S.q_lds) poa_call<NT, CPL, true, true>(a, A, S, R, adaptive);
+      else poa_call<NT, CPL, false, true>(a, A, S, R, adaptive);
       if (tid == 0 && a.call_stats && a.call_off[w] + r - 1 < a.call_off[w + 1]) {
         CallStat& cs = a.call_stats[a.call_off[w] + r - 1];
         cs.score = S.score;
@@ -2439,8 +2585,8 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
   a.scores = s.graph.scores;
   // -S always uses the adaptive band (align.cpp:1780, 1808, 1826).
   const bool adaptive = a.scores.f > 0;
-  if (S.q_lds) poa_call<NT, CPL, true, RingT>(a, A, S, R, adaptive);
-  else poa_call<NT, CPL, false, RingT>(a, A, S, R, adaptive);
+  if (S.q_lds) poa_call<NT, CPL, true, false, RingT>(a, A, S, R, adaptive);
+  else poa_call<NT, CPL, false, false, RingT>(a, A, S, R, adaptive);
   if (tid == 0) {
     out.status = S.status;
     out.score = S.score;
@@ -2569,8 +2715,8 @@ __global__ __launch_bounds__(NT) void k_graph_op(GraphOpArgs g) {
       return;
     }
     load_query<NT>(A, S, R, g.q, g.qlen);
-    if (S.q_lds) poa_call<NT, CPL, true, RingT>(a, A, S, R, g.adaptive != 0);
-    else poa_call<NT, CPL, false, RingT>(a, A, S, R, g.adaptive != 0);
+    if (S.q_lds) poa_call<NT, CPL, true, false, RingT>(a, A, S, R, g.adaptive != 0);
+    else poa_call<NT, CPL, false, false, RingT>(a, A, S, R, g.adaptive != 0);
     if (tid == 0) {
       GraphOpOut& o = *g.out;
       o.status = S.status;
@@ -2655,8 +2801,12 @@ __global__ __launch_bounds__(NT) void k_graph_op(GraphOpArgs g) {
   }
 }
 
+// The one-wavefront tile asks for three waves per SIMD (168 VGPRs at most): its LDS admits
+// eleven workgroups per CU, and the compiler's own occupancy target (two) would let the
+// register count grow past what three need. The larger tiles are held to two by their LDS.
 template <int NT, int CPL>
-__global__ __launch_bounds__(NT) void k_windows(KernelArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3 : 1, 8)))
+k_windows(KernelArgs a) {
   __shared__ WinShared S;
   __shared__ Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP> R;
   Arena A = carve(a, blockIdx.x);
