@@ -427,6 +427,14 @@ void assign_bins(mpoa_batch* b) {
   // One window (minipoa's own run) takes the 512-thread tile: its rows are the whole time.
   b->wide.nt = b->split ? 256 : 512;
   b->wide.cpl = 2;
+  // One window alone whose band passes the 512 threads' 1,024 columns: four columns a thread,
+  // so that every row is one pass (two passes cost twice the row's scan and barriers).
+  if (!b->split)
+    for (int w : b->wide.wins) {
+      const WinInfo& wi = b->info[w];
+      const int width = b->pred_width.empty() ? width_of(wi) : (int)b->pred_width[w];
+      if (width > 1024) b->wide.cpl = 4;
+    }
   b->bin_of.assign(b->n_windows, 0);
   // The costliest small-tile windows join the medium launch up to move_top windows (default
   // half a window per CU): they set the makespan, and the larger tile shortens their rows.

@@ -2893,7 +2893,8 @@ int mpoa_launch_noalloc(const mpoa::KernelArgs& a, const mpoa::WinDesc& wd, int 
 }
 
 #define MPOA_TILES(X) \
-  X(64, 1) X(64, 2) X(64, 4) X(64, 8) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4) X(512, 2)
+  X(64, 1) X(64, 2) X(64, 4) X(64, 8) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4) X(512, 2) \
+  X(512, 4)
 
 bool mpoa_tile_supported(int nt, int cpl) {
 #define MPOA_T(NT_, CPL_) if (nt == NT_ && cpl == CPL_) return true;
