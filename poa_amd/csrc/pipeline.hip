@@ -782,7 +782,10 @@ struct Ring {
   int row[RING];
   int bsbe[RING];
   uint8_t q[QCAP];         // shifted query codes when the read fits
-  int4 rec[3 * 32];        // row records of the current 32-row chunk
+  // Row records of the current 32-row chunk, with the fourth to sixth predecessors as
+  // (local rank << 3) | base in place of the in-weights (only the backtrack reads those, from
+  // the records in HBM).
+  int4 rec[3 * 32];
   static constexpr int kRing = RING, kWR = WR, kQ = QCAP;
 };
 
@@ -1089,15 +1092,26 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
                         meta_bsbe, sw);
     }
     if (rr.np > 3) {
-      for (int k = 3; k < rr.np; k++) {
-        const int p = A.pred_rank[rr.k0 + k] - beg;
-        const int pbase = A.pred_base[rr.k0 + k] & 7;
-        drain_vmem();
+      // the fourth to sixth from the record (LDS), later ones from HBM
+      const int n6 = min(rr.np, 6);
+      for (int k = 3; k < n6; k++) {
+        const int x = k == 3 ? rr.w0 : k == 4 ? rr.w1 : rr.w2;
+        const int p = x >> 3;
         if (p < 0) continue;
         npi++;
-        pred_contrib<CPL>(A, X, sc, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe, sw);
+        pred_contrib<CPL>(A, X, sc, p, x & 7, cb, c0, c1, qc, h, d, meta_row, meta_bsbe, sw);
       }
-      drain_vmem();
+      if (rr.np > 6) {
+        for (int k = 6; k < rr.np; k++) {
+          const int p = A.pred_rank[rr.k0 + k] - beg;
+          const int pbase = A.pred_base[rr.k0 + k] & 7;
+          drain_vmem();
+          if (p < 0) continue;
+          npi++;
+          pred_contrib<CPL>(A, X, sc, p, pbase, cb, c0, c1, qc, h, d, meta_row, meta_bsbe, sw);
+        }
+        drain_vmem();
+      }
     }
     int g[CPL], t[CPL];
     int lane_max = SCAN_ID;
@@ -1857,14 +1871,30 @@ __device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShare
   int wmax = 0;
   for (int i = 1; i < n; i++) {
     // Row records come through LDS, 32 rows at a time: one wait on HBM per chunk instead of
-    // one per row (a load's wait on gfx9 also waits for every store issued before it).
+    // one per row (a load's wait on gfx9 also waits for every store issued before it), and
+    // a second, shorter one (no store is pending then) for the chunk's fourth to sixth
+    // predecessors.
     if ((i - 1) % 32 == 0) {
       lds_barrier<NT>();
       if (tid < 32 && i + tid < n) {
         const int4* src = reinterpret_cast<const int4*>(A.rowrec + 12LL * (beg + i + tid));
-        R.rec[3 * tid] = src[0];
-        R.rec[3 * tid + 1] = src[1];
-        R.rec[3 * tid + 2] = src[2];
+        const int4 ra = src[0], rb = src[1];
+        int4 rc = src[2];
+        if (ra.w > 3) {
+          int x[3];
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            x[j] = 0;
+            if (3 + j < ra.w)
+              x[j] = ((A.pred_rank[ra.z + 3 + j] - beg) << 3) | (A.pred_base[ra.z + 3 + j] & 7);
+          }
+          rc.x = x[0];
+          rc.y = x[1];
+          rc.z = x[2];
+        }
+        R.rec[3 * tid] = ra;
+        R.rec[3 * tid + 1] = rb;
+        R.rec[3 * tid + 2] = rc;
       }
       lds_barrier<NT>();
     }
@@ -1904,16 +1934,26 @@ __device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShare
         if (sl >= 0) pmask |= 1ull << sl;
         else far = true;
       }
-      if (NT > 64 && rr.np > 3) {
-        for (int k = 3; k < rr.np; k++) {
-          const int p = A.pred_rank[rr.k0 + k] - beg;
-          drain_vmem();
-          if (p >= 0) far |= cache_slot(meta_row, p) < 0;
+      if (rr.np > 3) {
+        const int n6 = min(rr.np, 6);
+        for (int k = 3; k < n6; k++) {
+          const int p = (k == 3 ? rr.w0 : k == 4 ? rr.w1 : rr.w2) >> 3;
+          if (p < 0) continue;
+          const int sl = cache_slot(meta_row, p);
+          if (sl >= 0) pmask |= 1ull << sl;
+          else far = true;
+        }
+        if (NT > 64) {
+          for (int k = 6; k < rr.np; k++) {
+            const int p = A.pred_rank[rr.k0 + k] - beg;
+            drain_vmem();
+            if (p >= 0) far |= cache_slot(meta_row, p) < 0;
+          }
         }
       }
       if (NT > 64 && far) __syncthreads();
     }
-    // A slot for this row: a free one, else (predecessors after the third unknown here: no
+    // A slot for this row: a free one, else (predecessors after the sixth unknown here: no
     // eviction then) the one whose row is read last.
     const int last_i = rr.last < 0 ? -1 : rr.last - beg;
     int slot_i = -1;
@@ -1921,7 +1961,7 @@ __device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShare
       const uint64_t freem = __ballot(lane < ns && meta_last < i);
       if (freem) {
         slot_i = (int)__builtin_ctzll(freem);
-      } else if (rr.np <= 3) {
+      } else if (rr.np <= 6) {
         const int v = (lane < ns && !((pmask >> lane) & 1)) ? meta_last : -1;
         int mx = v;
 #pragma unroll
