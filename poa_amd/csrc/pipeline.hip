@@ -810,6 +810,18 @@ __device__ __forceinline__ int wave_incl_max(int v) {
   return v;
 }
 
+// Inclusive prefix min across a wavefront (the same DPP steps; identity INT32_MAX).
+__device__ __forceinline__ int wave_incl_min(int v) {
+  constexpr int ID = 0x7fffffff;
+  v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return v;
+}
+
 // Exclusive prefix max of v over the workgroup's threads, and the maximum over all of them.
 // The wave partials alternate between two halves of S.scan (ph, flipped by the caller on
 // every call): a call's reads of one half are then ordered before the next write to it by the
@@ -1151,11 +1163,12 @@ __device__ __forceinline__ int dp_row(const Arena& A, WinShared& S, RingT& R, co
   // NEG_INF sentinel block after the band (align.cpp:1505-1506,1515).
   if (tid < Q) Hi[W + tid] = Di[W + tid] = Ii[W + tid] = store_of<T>(NEG_INF);
   if (!want_argmax) return 0;
-  // First argmax over the row (value desc, column asc).
-#pragma unroll
-  for (int s = 32; s > 0; s >>= 1) {
-    const int ov = __shfl_xor(best, s, 64), oj = __shfl_xor(best_j, s, 64);
-    if (ov > best || (ov == best && oj < best_j)) { best = ov; best_j = oj; }
+  // First argmax over the row (value desc, column asc): the wave's maximum, then the smallest
+  // column holding it (two DPP reductions).
+  {
+    const int own = best;
+    best = __builtin_amdgcn_readlane(wave_incl_max(own), 63);
+    best_j = __builtin_amdgcn_readlane(wave_incl_min(own == best ? best_j : 0x7fffffff), 63);
   }
   if (NT > 64) {
     const int wave = tid >> 6;
@@ -1962,11 +1975,10 @@ __device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShare
       if (freem) {
         slot_i = (int)__builtin_ctzll(freem);
       } else if (rr.np <= 6) {
+        // the latest last reader among the slots this row does not read: a DPP wave maximum
+        // (shuffles would be six LDS round trips on every row once the cache is full)
         const int v = (lane < ns && !((pmask >> lane) & 1)) ? meta_last : -1;
-        int mx = v;
-#pragma unroll
-        for (int sh = 32; sh > 0; sh >>= 1) mx = max(mx, __shfl_xor(mx, sh, 64));
-        mx = __builtin_amdgcn_readfirstlane(mx);
+        const int mx = __builtin_amdgcn_readlane(wave_incl_max(v), 63);
         if (mx > last_i) slot_i = (int)__builtin_ctzll(__ballot(v == mx));
       }
     }
