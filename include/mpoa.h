@@ -335,15 +335,27 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
  * predecessors); MPOA_FORM_ADD_LANE, the one-lane add_path (a repeated edge);
  * MPOA_FORM_CONS_ROUNDS, the -S consensus settling every block in rounds (a rank with more than
  * eight predecessors in its block); MPOA_FORM_WIDE, int32 storage of the DP matrices for every
- * call (a call whose values do not fit int16: minipoa's own width). 0 (the default) restores
+ * call (a call whose values do not fit int16: minipoa's own width); MPOA_FORM_HOST_SEED, -S
+ * reads seeded on the host between launches (a window that outgrows the device seeding's
+ * capacities, or ties among more than 64 equal minimizer or anchor keys). 0 (the default) restores
  * the normal choice. Results do not change; only speed does. mpoa_set_forms sets the forms of
  * the batches the context creates afterwards, mpoa_align_reads' included. */
 enum {
   MPOA_FORM_SORT_HBM = 1, MPOA_FORM_SORT_LANE = 2, MPOA_FORM_BT_LANE = 4, MPOA_FORM_ADD_LANE = 8,
-  MPOA_FORM_CONS_ROUNDS = 16, MPOA_FORM_WIDE = 32
+  MPOA_FORM_CONS_ROUNDS = 16, MPOA_FORM_WIDE = 32, MPOA_FORM_HOST_SEED = 64
 };
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms);
 int mpoa_set_forms(mpoa_ctx* ctx, int32_t forms);
+/* Diagnostics (no counterpart in minipoa): how the last seeded (-S) mpoa_align_reads call of
+ * the context was seeded: 1 on the device (every read seeded by the workgroup that committed the
+ * read before it, no host round trip); 0 on the host by choice (MPOA_FORM_HOST_SEED, fewer than
+ * two reads, -k above 28 or -w above 255); above 1 on the host after the device form stopped,
+ * the value naming why as bits: 2 the graph outgrew its estimate, 4 a read's minimizers, 8 the
+ * consensus length, 16 its minimizers, 32 equal keys among more than 64 consensus minimizers,
+ * 64 more than 4,096 anchors, 128 / 256 / 512 equal keys among more than 64 anchors / chain ends
+ * / chains (minipoa's radix sort may order them otherwise), 1024 more than 32 sub-windows, 2048
+ * their DP pool, 4096 anchor-base splices; -1 before any seeded call. */
+int mpoa_last_seeding(const mpoa_ctx* ctx);
 
 /* Graph replay of the batched pass (default on): the concurrent persistent launches, each with
  * its work-queue reset, are captured once into a hipGraph and replayed by every later

@@ -21,6 +21,7 @@ MPOA_OK = 0
 MPOA_E_OUT_CAP = -4
 FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE, FORM_CONS_ROUNDS = 1, 2, 4, 8, 16
 FORM_WIDE = 32  # int32 storage of the DP matrices for every call (no int16 form)
+FORM_HOST_SEED = 64  # -S reads seeded on the host between launches (no device seeding)
 ERRORS = {
     -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
     -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW", -9: "E_SCRATCH",
@@ -31,7 +32,7 @@ EXPORTS = [
     "mpoa_batch_execute", "mpoa_batch_results", "mpoa_batch_free", "mpoa_batch_set_tile",
     "mpoa_batch_window_stats", "mpoa_batch_launch_stats", "mpoa_batch_set_traces",
     "mpoa_batch_traces", "mpoa_batch_set_mode", "mpoa_batch_set_graph", "mpoa_batch_set_forms",
-    "mpoa_set_forms", "mpoa_align_reads_noalloc", "mpoa_graph_create", "mpoa_graph_destroy",
+    "mpoa_set_forms", "mpoa_last_seeding", "mpoa_align_reads_noalloc", "mpoa_graph_create", "mpoa_graph_destroy",
     "mpoa_poa", "mpoa_seeder_create", "mpoa_seeder_destroy", "mpoa_seeder_order",
     "mpoa_alignment",
     "mpoa_graph_add_path", "mpoa_graph_topsort", "mpoa_graph_output", "mpoa_graph_export",
@@ -132,6 +133,7 @@ def lib() -> ctypes.CDLL:
         L.mpoa_batch_set_graph.argtypes = [VP, I32]
         L.mpoa_batch_set_forms.argtypes = [VP, I32]
         L.mpoa_set_forms.argtypes = [VP, I32]
+        L.mpoa_last_seeding.argtypes = [VP]
         L.mpoa_batch_set_schedule.argtypes = [VP, I32, I32, I32, I32]
         L.mpoa_batch_set_predictions.argtypes = [VP, VP, I32]
         L.mpoa_batch_traces.argtypes = [VP, VP, I64, P(I64)]
@@ -202,6 +204,12 @@ class Context:
         """Test hook: the FORM_* fallback forms of every batch created afterwards, including
         the one behind align_reads; 0 restores the normal choice. Results do not change."""
         _check(lib().mpoa_set_forms(self.h, int(forms)))
+
+    def last_seeding(self) -> int:
+        """How the last seeded align_reads call was seeded: 1 on the device, 0 on the host by
+        choice, above 1 on the host after the device form stopped (bits naming why, as
+        mpoa_last_seeding documents), -1 before any seeded call."""
+        return int(lib().mpoa_last_seeding(self.h))
 
     def align_reads(self, params: Params, seq: np.ndarray, off: np.ndarray,
                     names: list[str] | None = None, comments: list[str] | None = None,

@@ -25,6 +25,7 @@ struct mpoa_ctx {
   std::string held_key, held_text;
   std::vector<mpoa_call_stat> held_calls;
   int forms = 0;  // MPOA_FORM_* test hook: the forms of batches created afterwards
+  int last_seeding = -1;  // the last seeded mpoa_align_reads: 1 device seeding, 0 host, -1 none
   uint8_t* scratch = nullptr;  // this device's allocation-free scratch array (device address)
 };
 

@@ -580,6 +580,178 @@ int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vecto
   return MPOA_OK;
 }
 
+// A seeded (-S) window seeded on the device (SeedDev): one launch sketches every read, then per
+// read the launch of its sub-windows and its commit, whose workgroup seeds the next read
+// against the new consensus. Everything is enqueued at once and synchronised once. Returns 1
+// when the window outgrew a capacity of this form (seeding buffers, sub-window count or pool,
+// graph estimate; run_seeded below then runs it), else the window's status.
+int run_seeded_dev(mpoa_batch* b, const int64_t* seq_off, std::vector<mpoa_call_stat>* calls) {
+  using mpoa::SeedDev;
+  mpoa_ctx* ctx = b->ctx;
+  const WinInfo& wi = b->info[0];
+  const int n = wi.n_reads;
+  const int k = b->params.k, w = b->params.mm_w;
+  if (n < 2 || k < 1 || k > 28 || w < 1 || w > 255) {
+    ctx->last_seeding = 0;
+    return 1;
+  }
+  const bool keep_path = b->params.result != 0;
+  const Estimate e = estimate(wi, b->sc, true);
+  SlotCaps gc{};
+  gc.nodes = (std::max(e.nodes, 2) + 63) / 64 * 64;
+  gc.edges = (std::max(e.edges, 4) + 63) / 64 * 64;
+  gc.qmax = std::max(1, wi.qmax);
+  gc.path = keep_path ? (int)wi.qtot : 0;
+  gc.dp = 64;
+  mpoa::finish_caps(gc);
+  SlotCaps sc_caps = gc;
+  sc_caps.edges = 64;
+  sc_caps.path = 0;
+  sc_caps.dp = 0;  // every sub-window takes its share of one pool (SegDesc::dp0)
+  mpoa::finish_caps(sc_caps);
+  // One read's sub-windows: a band row per rank between two anchors, about a min_w-th of
+  // the read wide; dp_cap allows every rank an eighth of the longest read.
+  const int64_t dp_cap =
+      std::max<int64_t>(3LL * (16 << 20), 3LL * (gc.nodes + 64) * (gc.qmax / 8 + 64));
+  const int cons_cap = gc.nodes;
+  std::vector<int64_t> rm_off(n + 1, 0);
+  for (int r = 0; r < n; r++)
+    rm_off[r + 1] = rm_off[r] + (seq_off[r + 1] - seq_off[r]) + 2 * w + 8;
+  DevBuf<WinState> d_state;
+  DevBuf<int> g_int, g_dp, s_int, s_dp, i_buf;
+  DevBuf<uint8_t> g_byte, s_byte;
+  DevBuf<uint64_t> u_buf;
+  DevBuf<int64_t> d_rm_off;
+  DevBuf<SegDesc> d_segs;
+  DevBuf<SegResult> d_segres;
+  int rc;
+  const size_t rm = (size_t)rm_off[n], hx = (size_t)wi.qtot + cons_cap;
+  const size_t u_words = 4 * rm + hx + 4 * (size_t)cons_cap + 8 * (size_t)mpoa::SEED_ANCHORS;
+  // ints: rm_cnt, n_seg, n_splice, read_graph (2 per read), splices, flag
+  const size_t i_words = 5 * (size_t)n + 2 * (size_t)n * mpoa::SEED_SPLICE + 1;
+  if ((rc = d_state.alloc(1)) != MPOA_OK || (rc = g_int.alloc(gc.ints_per_slot)) != MPOA_OK ||
+      (rc = g_byte.alloc(gc.bytes_per_slot)) != MPOA_OK || (rc = g_dp.alloc(3 * 64)) != MPOA_OK ||
+      (rc = s_int.alloc(mpoa::SEED_SEGS * sc_caps.ints_per_slot)) != MPOA_OK ||
+      (rc = s_byte.alloc(mpoa::SEED_SEGS * sc_caps.bytes_per_slot)) != MPOA_OK ||
+      (rc = s_dp.alloc((size_t)dp_cap)) != MPOA_OK || (rc = u_buf.alloc(u_words)) != MPOA_OK ||
+      (rc = i_buf.alloc(i_words)) != MPOA_OK || (rc = d_rm_off.alloc(n + 1)) != MPOA_OK ||
+      (rc = d_segs.alloc((size_t)n * mpoa::SEED_SEGS)) != MPOA_OK ||
+      (rc = d_segres.alloc((size_t)n * mpoa::SEED_SEGS)) != MPOA_OK)
+    return rc;
+  HIPCHK(hipMemcpyAsync(d_rm_off.p, rm_off.data(), (n + 1) * sizeof(int64_t),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(i_buf.p, 0, i_words * sizeof(int), ctx->stream));
+  SeedDev d{};
+  uint64_t* u = u_buf.p;
+  auto take_u = [&](size_t m) { uint64_t* p = u; u += m; return p; };
+  d.rmx = take_u(rm); d.rmy = take_u(rm); d.rtx = take_u(rm); d.rty = take_u(rm);
+  d.hx = take_u(hx);
+  d.cmx = take_u(cons_cap); d.cmy = take_u(cons_cap);
+  d.ctx = take_u(cons_cap); d.cty = take_u(cons_cap);
+  d.ax = take_u(mpoa::SEED_ANCHORS); d.ay = take_u(mpoa::SEED_ANCHORS);
+  d.bx = take_u(mpoa::SEED_ANCHORS); d.by = take_u(mpoa::SEED_ANCHORS);
+  d.zx = take_u(mpoa::SEED_ANCHORS); d.zy = take_u(mpoa::SEED_ANCHORS);
+  d.zsx = take_u(mpoa::SEED_ANCHORS); d.zsy = take_u(mpoa::SEED_ANCHORS);
+  int* iw = i_buf.p;
+  d.rm_cnt = iw; iw += n;
+  d.n_seg = iw; iw += n;
+  d.n_splice = iw; iw += n;
+  d.read_graph = iw; iw += 2 * n;
+  d.splice = iw; iw += 2 * (size_t)n * mpoa::SEED_SPLICE;
+  d.flag = iw;
+  d.rm_off = d_rm_off.p;
+  d.hx_cons = wi.qtot;
+  d.cons_cap = cons_cap;
+  d.segs = d_segs.p;
+  d.dp_cap = dp_cap;
+  d.k = k;
+  d.w = w;
+  d.bw = 1000;  // main.cpp:65
+  d.poa_w = b->params.poa_w;
+  d.pen_gap = (float)(0.01 * k);  // minimizer.cpp:498
+  d.pen_skip = (float)(0.0 * 0.01 * k);
+  KernelArgs ga{};
+  ga.forms = b->forms;
+  ga.scores = b->sc;
+  ga.caps = gc;
+  ga.read_seq = b->d_seq.p;
+  ga.read_off = b->d_off.p;
+  ga.win = b->d_win.p;
+  ga.result = b->d_res.p;
+  ga.n_jobs = 1;
+  ga.int_pool = g_int.p;
+  ga.byte_pool = g_byte.p;
+  ga.dp_pool = g_dp.p;
+  ga.out = b->d_out.p;
+  ga.gfa_lines = b->d_gfa.p;
+  SeedArgs sa{};
+  sa.graph = ga;
+  sa.seg = ga;
+  sa.seg.caps = sc_caps;
+  sa.seg.int_pool = s_int.p;
+  sa.seg.byte_pool = s_byte.p;
+  sa.seg.dp_pool = s_dp.p;
+  sa.n_reads = n;
+  sa.state = d_state.p;
+  sa.use_dev = 1;
+  sa.dev = d;
+  if (mpoa_launch_seed_reads(sa, ctx->stream) != 0) {
+    g_err = "read sketch launch failed";
+    return MPOA_E_DEVICE;
+  }
+  int path0 = 0;
+  for (int r = 0; r < n; r++) {
+    sa.read = r;
+    sa.path0 = path0;
+    sa.segs = d.segs + (size_t)r * mpoa::SEED_SEGS;
+    sa.seg_res = d_segres.p + (size_t)r * mpoa::SEED_SEGS;
+    sa.splice = d.splice + (size_t)r * 2 * mpoa::SEED_SPLICE;
+    if (r > 0 && mpoa_launch_segments(sa, 128, 4, ctx->stream) != 0) {
+      g_err = "sub-window launch failed";
+      return MPOA_E_DEVICE;
+    }
+    if (mpoa_launch_commit(sa, ctx->stream) != 0) {
+      g_err = "commit launch failed";
+      return MPOA_E_DEVICE;
+    }
+    b->launches += r > 0 ? 2 : 1;
+    path0 += (int)(seq_off[r + 1] - seq_off[r]);
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<int> iv(i_words);
+  HIPCHK(hipMemcpy(iv.data(), i_buf.p, i_words * sizeof(int), hipMemcpyDeviceToHost));
+  b->res.assign(1, WinResult{});
+  HIPCHK(hipMemcpy(b->res.data(), b->d_res.p, sizeof(WinResult), hipMemcpyDeviceToHost));
+  const int* h_nseg = iv.data() + n;
+  const int* h_graph = iv.data() + 3 * n;
+  if (iv[i_words - 1] != 0 || b->res[0].status == MPOA_E_OVERFLOW) {
+    ctx->last_seeding = iv[i_words - 1] != 0 ? iv[i_words - 1] : mpoa::SEED_WHY_GRAPH;
+    return 1;
+  }
+  ctx->last_seeding = 1;
+  calls->clear();
+  std::vector<SegResult> sr((size_t)n * mpoa::SEED_SEGS);
+  HIPCHK(hipMemcpy(sr.data(), d_segres.p, sr.size() * sizeof(SegResult), hipMemcpyDeviceToHost));
+  for (int r = 1; r < n; r++) {
+    for (int s = 0; s < h_nseg[r]; s++) {
+      const SegResult& g = sr[(size_t)r * mpoa::SEED_SEGS + s];
+      mpoa_call_stat c{};
+      c.read = r;
+      c.rows = g.rows;
+      c.qlen = g.qlen;
+      c.score = g.score;
+      c.cells = g.cells;
+      c.edge_cells = g.edge_cells;
+      c.logical_cells = g.logical;
+      c.width = g.width;
+      c.nodes = h_graph[2 * r];
+      c.edges = h_graph[2 * r + 1];
+      calls->push_back(c);
+    }
+  }
+  return b->res[0].status;
+}
+
 // A seeded (-S) window, read by read (main.cpp:110-121 with alignment()'s -S branch,
 // align.cpp:1763-1827). The host seeds each read against the consensus the device built
 // after the previous read, splits the read at the chain's anchors, and launches one
@@ -1304,13 +1476,16 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode) {
 }
 
 constexpr int kAllForms = MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE | MPOA_FORM_BT_LANE |
-                          MPOA_FORM_ADD_LANE | MPOA_FORM_CONS_ROUNDS | MPOA_FORM_WIDE;
+                          MPOA_FORM_ADD_LANE | MPOA_FORM_CONS_ROUNDS | MPOA_FORM_WIDE |
+                          MPOA_FORM_HOST_SEED;
 
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms) {
   if (!b || (forms & ~kAllForms) != 0) return MPOA_E_ARG;
   b->forms = forms;
   return MPOA_OK;
 }
+
+int mpoa_last_seeding(const mpoa_ctx* ctx) { return ctx ? ctx->last_seeding : -1; }
 
 int mpoa_set_forms(mpoa_ctx* ctx, int32_t forms) {
   if (!ctx || (forms & ~kAllForms) != 0) return MPOA_E_ARG;
@@ -1499,7 +1674,10 @@ int mpoa_align_reads(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads, 
   if (seeding) {
     // with -p the batch holds the reads in guide-tree order; the seeding follows that order
     const bool perm = !b->row_of.empty();
-    rc = run_seeded(b, perm ? b->perm_seq.data() : seq, perm ? b->perm_off.data() : so, &calls);
+    const int64_t* po = perm ? b->perm_off.data() : so;
+    ctx->last_seeding = 0;
+    rc = (ctx->forms & MPOA_FORM_HOST_SEED) ? 1 : run_seeded_dev(b, po, &calls);
+    if (rc == 1) rc = run_seeded(b, perm ? b->perm_seq.data() : seq, po, &calls);
   } else {
     b->split = false;  // one window: the wide-tile launch
     assign_bins(b);

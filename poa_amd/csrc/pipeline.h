@@ -163,8 +163,11 @@ struct KernelArgs {
 
 // -S sub-window of the current read: the DP of read characters [q0, q0 + qlen) against the
 // graph's rank window [rank(beg_id), rank(end_id)]; its alignment goes to res [res0, ...).
+// With dp_len > 0 its H, D and I (dp_len int32 each) lie at dp_pool + dp0 instead of the slot's
+// share of the pool (the sub-windows a read's device seeding carves out of one pool).
 struct SegDesc {
   int beg_id, end_id, q0, qlen, res0, pad;
+  int64_t dp0, dp_len;
 };
 
 struct SegResult {
@@ -180,6 +183,49 @@ struct WinState {
   int64_t ticks[PH_N];
 };
 
+// Capacities of device seeding (per read): sub-windows, anchors kept for chaining, entries one
+// LDS sort takes, and anchor-base splices.
+constexpr int SEED_SEGS = 32;
+constexpr int SEED_ANCHORS = 4096;
+constexpr int SEED_SORT = 8192;
+constexpr int SEED_SPLICE = 1024;
+// Why a window left device seeding (SeedDev::flag bits; bit 1 is the graph estimate's, set by
+// the host): read minimizers past capacity or with ties, consensus past cons_cap, its
+// minimizers past cons_cap, ties in their sort, anchors past SEED_ANCHORS, ties among them or
+// among chain ends or chains, sub-windows past SEED_SEGS, their pool, splices.
+enum {
+  SEED_WHY_GRAPH = 2, SEED_WHY_READS = 4, SEED_WHY_CONS = 8, SEED_WHY_CONS_MM = 16,
+  SEED_WHY_CONS_SORT = 32, SEED_WHY_ANCHORS = 64, SEED_WHY_ANCHOR_SORT = 128,
+  SEED_WHY_END_SORT = 256, SEED_WHY_CHAIN_SORT = 512, SEED_WHY_SEGS = 1024,
+  SEED_WHY_POOL = 2048, SEED_WHY_SPLICE = 4096
+};
+
+// Device seeding of one -S window (minipoa's minimizer.cpp / lchain.cpp steps on the GPU): the
+// reads' minimizers sorted by hash, once; after each read's commit, the consensus's minimizers,
+// the anchors of the next read, its chains and its sub-windows. Buffers in HBM, sized by the
+// host; a step that outgrows them sets *flag and stops the window (the host then seeds it).
+struct SeedDev {
+  uint64_t *rmx, *rmy;      // read minimizers sorted by x: read r at [rm_off[r], + rm_cnt[r])
+  uint64_t *rtx, *rty;      // the same before sorting
+  const int64_t* rm_off;    // n_reads + 1 capacity offsets
+  int* rm_cnt;              // n_reads
+  uint64_t* hx;             // x per position: the window's read bases, then cons_cap more
+  int64_t hx_cons;          // first consensus position in hx
+  int cons_cap;             // consensus positions and minimizers kept
+  uint64_t *cmx, *cmy, *ctx, *cty;  // consensus minimizers: sorted / as emitted (cons_cap)
+  uint64_t *ax, *ay, *bx, *by;      // anchors: joined / sorted (SEED_ANCHORS)
+  uint64_t *zx, *zy, *zsx, *zsy;    // chain ends, then chains: unsorted / sorted (SEED_ANCHORS)
+  SegDesc* segs;            // SEED_SEGS per read
+  int* splice;              // SEED_SPLICE (position, node id) pairs per read
+  int* n_seg;               // per read (0: the read is added as a new path)
+  int* n_splice;            // per read
+  int* read_graph;          // per read: nodes and in-edges after its commit
+  int* flag;                // nonzero: a capacity was exceeded; the host seeds this window
+  int64_t dp_cap;           // int32 words of dp_pool for one read's sub-windows (H, D, I each)
+  int k, w, bw, poa_w;
+  float pen_gap, pen_skip;
+};
+
 struct SeedArgs {
   KernelArgs graph;        // the window (win[0]) and the graph in slot 0 of its pools
   KernelArgs seg;          // private scratch per sub-window (slot = sub-window)
@@ -193,6 +239,8 @@ struct SeedArgs {
   WinState* state;
   int* cons_out;           // after a commit: (node id, rank, base) per consensus position
   int cons_cap;
+  int use_dev;             // 1: sub-window counts from dev, and the commit seeds the next read
+  SeedDev dev;
 };
 
 // One step of minipoa's per-read loop on a graph kept in device memory between calls (the
@@ -249,6 +297,8 @@ int mpoa_graph_add_hold(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
 // read (one workgroup). 0, -1 on a launch error, -2 for an unsupported tile.
 int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream);
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream);
+// Device seeding: every read's minimizers (one workgroup per read), before the first commit.
+int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream);
 // Device address of this device's allocation-free scratch array (MPOA_NOALLOC_SCRATCH_BYTES),
 // looked up once per device with that device current. 0, or -1 when the lookup fails.
 int mpoa_scratch_address(int device, void** out);

@@ -20,6 +20,7 @@
 #include <mutex>
 
 #include "pipeline.h"
+#include "seed_core.h"
 
 namespace mpoa {
 
@@ -2568,6 +2569,8 @@ __device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinSha
 // and the window state between launches in SeedArgs::state.
 // ---------------------------------------------------------------------------------------
 
+#include "seed_dev.h"
+
 __device__ void state_load(WinShared& S, const WinState& st) {
   S.n_nodes = st.n_nodes; S.n_in_e = st.n_in_e; S.n_out_e = st.n_out_e;
   S.status = st.status; S.msa_cols = st.msa_cols; S.cons_len = st.cons_len;
@@ -2587,9 +2590,15 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
   __shared__ Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP> R;
   using RingT = Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP>;
   const int tid = threadIdx.x;
+  if (s.use_dev && (int)blockIdx.x >= s.dev.n_seg[s.read]) return;
   const SegDesc d = s.segs[blockIdx.x];
   const Arena G = carve(s.graph, 0);
   Arena A = carve(s.seg, blockIdx.x);
+  if (d.dp_len > 0) {  // the sub-window's own share of the pool (device seeding)
+    A.H = s.seg.dp_pool + d.dp0;
+    A.D = A.H + d.dp_len;
+    A.I = A.D + d.dp_len;
+  }
   // The graph is read-only here; the DP plan, band state, query and matrices are private.
   A.par = G.par; A.aligned = G.aligned; A.ind = G.ind; A.rank = G.rank;
   A.in_head = G.in_head; A.in_tail = G.in_tail; A.out_head = G.out_head;
@@ -2635,6 +2644,7 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
   load_query<NT>(A, S, R, s.graph.read_seq + q0, d.qlen);
   KernelArgs a = s.seg;
   a.scores = s.graph.scores;
+  if (d.dp_len > 0) a.caps.dp = d.dp_len;
   // -S always uses the adaptive band (align.cpp:1780, 1808, 1826).
   const bool adaptive = a.scores.f > 0;
   if (S.q_lds) poa_call<NT, CPL, true, false, RingT>(a, A, S, R, adaptive);
@@ -2670,10 +2680,12 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
   const int64_t q0 = a.read_off[wd.read0 + r], q1 = a.read_off[wd.read0 + r + 1];
   const int qlen = (int)(q1 - q0), m = qlen + 1;
   const bool keep_path = a.scores.result != 0;
+  const int n_seg = s.use_dev ? s.dev.n_seg[r] : s.n_seg;
+  const int n_splice = s.use_dev ? s.dev.n_splice[r] : s.n_splice;
   if (S.status == MPOA_OK) {
     if (qlen > a.caps.qmax || (keep_path && s.path0 + qlen > a.caps.path)) {
       if (tid == 0) S.status = MPOA_E_OVERFLOW;
-    } else if (s.n_seg == 0) {
+    } else if (n_seg == 0) {
       for (int j = tid; j < m; j += NT) {
         A.res_from[j] = j == 0 ? 0 : -1;
         A.res_base[j] = j == 0 ? (uint8_t)CODE_N : (uint8_t)code_of(a.read_seq[q0 + j - 1]);
@@ -2681,7 +2693,7 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
       }
     } else {
       if (tid == 0) {
-        for (int k = 0; k < s.n_seg; k++) {
+        for (int k = 0; k < n_seg; k++) {
           const SegResult& g = s.seg_res[k];
           if (g.status != MPOA_OK && S.status == MPOA_OK) S.status = g.status;
           S.cells += g.cells;
@@ -2691,7 +2703,7 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
         }
       }
       // Anchor bases between sub-windows: existing consensus nodes (align.cpp:1794-1797).
-      for (int k = tid; k < s.n_splice; k += NT) {
+      for (int k = tid; k < n_splice; k += NT) {
         const int pos = s.splice[2 * k], id = s.splice[2 * k + 1];
         A.res_from[pos] = id;
         A.res_base[pos] = A.base[id];
@@ -2704,7 +2716,13 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
     commit_read<NT>(a, A, S, r, m, s.path0, r + 1 == s.n_reads, true,
                     reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
   }
-  if (S.status == MPOA_OK && r + 1 < s.n_reads) {
+  if (s.use_dev && tid == 0) {
+    s.dev.read_graph[2 * r] = S.n_nodes;
+    s.dev.read_graph[2 * r + 1] = S.n_in_e;
+  }
+  if (s.use_dev && S.status == MPOA_OK && r + 1 < s.n_reads) {
+    seed_next<NT>(s, A, S, reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
+  } else if (S.status == MPOA_OK && r + 1 < s.n_reads) {
     if (S.cons_len > s.cons_cap) {
       if (tid == 0) S.status = MPOA_E_OVERFLOW;
     } else {
@@ -3024,15 +3042,24 @@ int mpoa_graph_add_hold(hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
 
 int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t stream) {
   using namespace mpoa;
-  if (s.n_seg <= 0) return 0;
+  // device seeding: as many workgroups as a read may have sub-windows; the extra ones return
+  const int grid = s.use_dev ? SEED_SEGS : s.n_seg;
+  if (grid <= 0) return 0;
 #define MPOA_S(NT_, CPL_)                                                                     \
   if (nt == NT_ && cpl == CPL_) {                                                            \
-    hipLaunchKernelGGL((k_segments<NT_, CPL_>), dim3(s.n_seg), dim3(NT_), 0, stream, s);     \
+    hipLaunchKernelGGL((k_segments<NT_, CPL_>), dim3(grid), dim3(NT_), 0, stream, s);        \
     return hipGetLastError() == hipSuccess ? 0 : -1;                                         \
   }
   MPOA_S(64, 4) MPOA_S(128, 4) MPOA_S(256, 2)
 #undef MPOA_S
   return -2;
+}
+
+int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream) {
+  using namespace mpoa;
+  if (s.n_reads <= 0) return 0;
+  hipLaunchKernelGGL((k_seed_reads<256>), dim3(s.n_reads), dim3(256), 0, stream, s);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream) {

@@ -175,7 +175,7 @@ def test_fallback_forms_reject_unknown_bits(ctx):
     b = synth.config3(seed=26, n_windows=2)
     bat = api.Batch(ctx, api.Params(), b.win_off, b.seq, b.seq_off)
     with pytest.raises(api.MpoaError):
-        bat.set_forms(64)
+        bat.set_forms(128)
     bat.free()
 
 

@@ -1,0 +1,130 @@
+"""Device seeding of -S windows: every read seeded by the workgroup that committed the read
+before it (minimizers, anchors, local and global chains, sub-windows on the GPU), against the
+host-seeded form (MPOA_FORM_HOST_SEED) and upstream's outputs. Both forms must split every read
+into the same sub-windows: the per-call (read, rows, qlen, score) lists are compared whole."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import full_util as fu
+import golden_util as gu
+import oracle_lib
+from poa_amd import api, synth
+
+
+def _run(ctx, params, seq, off, forms, names=None, comments=None):
+    ctx.set_forms(forms)
+    try:
+        out, stats = ctx.align_reads(params, seq, off, names, comments, with_stats=True)
+        how = ctx.last_seeding()
+    finally:
+        ctx.set_forms(0)
+    return out, [(s.read, s.rows, s.qlen, s.score) for s in stats], how
+
+
+def _outcome(ctx, params, seq, off, forms):
+    """(text or None, calls, seeding form, status): minipoa's failures (an empty sub-window,
+    a backtrack error) are statuses to compare, not test errors."""
+    try:
+        out, calls, how = _run(ctx, params, seq, off, forms)
+        return out, calls, how, 0
+    except api.MpoaError as e:
+        return None, None, ctx.last_seeding(), e.rc
+
+
+def _oracle(params, seq, off):
+    try:
+        return oracle_lib.window(params, seq, off)[0], 0
+    except RuntimeError as e:  # "oracle error <status>"
+        return None, int(str(e).split()[-1])
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+SEEDED = [c for c in fu.load_cases() if "-S" in c["args"]]
+EMPTY = -5  # MPOA_E_EMPTY_WINDOW
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", SEEDED, ids=[c["case"] for c in SEEDED])
+def test_full_configs_seeded_on_device(ctx, case):
+    reads = fu.reads_of(case)
+    _, text = fu.fasta_of(reads)
+    seq, off, names, comments = gu.parse_fasta(text)
+    p = api.Params(**gu.parse_args(case["args"]))
+    _say("seed_dev", case["case"])
+    out, calls, how = _run(ctx, p, seq, off, 0, names, comments)
+    assert how == 1, f"the window left device seeding ({how})"
+    assert out == case["stdout"]
+    assert [c[3] for c in calls] == case["scores"]
+    out_h, calls_h, how_h = _run(ctx, p, seq, off, api.FORM_HOST_SEED, names, comments)
+    assert how_h == 0
+    assert out_h == out
+    assert calls_h == calls
+
+
+def _windows(seed: int, n: int):
+    b = synth.config3(seed=seed, n_windows=n)
+    for w in range(n):
+        r0, r1 = int(b.win_off[w]), int(b.win_off[w + 1])
+        o = b.seq_off[r0:r1 + 1]
+        yield w, b.seq[o[0]:o[-1]].copy(), (o - o[0]).astype(np.int64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flags", [dict(seeding=1, result=0), dict(seeding=1, result=1),
+                                   dict(seeding=1, k=15, mm_w=5, result=0),
+                                   dict(seeding=1, k=11, mm_w=3, poa_w=40, result=1)],
+                         ids=["S_r0", "S_r1", "S_k15w5", "S_k11w3W40"])
+def test_random_windows_against_oracle(ctx, flags):
+    p = api.Params(**flags)
+    for w, seq, off in _windows(31 + flags.get("k", 19), 12):
+        _say("seed_dev window", w, flags)
+        want, want_rc = _oracle(p, seq, off)
+        out, calls, how, rc = _outcome(ctx, p, seq, off, 0)
+        assert how == 1, f"window {w} left device seeding ({how})"
+        if rc == EMPTY and want_rc != 0:
+            # anchors closer than k on the read: a sub-window with a negative query length,
+            # on which minipoa runs poa() past its query's end and exits one way or another
+            pass
+        else:
+            assert (rc, out) == (want_rc, want), f"window {w}"
+        out_h, calls_h, how_h, rc_h = _outcome(ctx, p, seq, off, api.FORM_HOST_SEED)
+        assert how_h == 0
+        assert (rc_h, out_h, calls_h) == (rc, out, calls), f"window {w}"
+
+
+def _repeat_reads(seed: int, n_reads: int, unit: int, copies: int):
+    """Reads of a template holding a tandem repeat (identical minimizers within each read)."""
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    rep = np.tile(rng.integers(0, 4, unit), copies)
+    tpl = np.concatenate([rng.integers(0, 4, 300), rep, rng.integers(0, 4, 300)])
+    seqs = []
+    for _ in range(n_reads):
+        s = tpl.copy()
+        hit = rng.random(len(s)) < 0.03
+        s[hit] = rng.integers(0, 4, int(hit.sum()))
+        seqs.append(acgt[s])
+    off = np.zeros(n_reads + 1, np.int64)
+    off[1:] = np.cumsum([len(s) for s in seqs])
+    return np.concatenate(seqs), off
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("unit,copies", [(40, 3), (90, 12)])
+def test_repeats_against_oracle(ctx, unit, copies):
+    """Equal minimizer keys within reads and consensus: the device repeats minipoa's radix
+    sort steps for them (the order of equal keys decides which anchors chain)."""
+    seq, off = _repeat_reads(7 + unit, 8, unit, copies)
+    p = api.Params(seeding=1, result=1)
+    _say("seed_dev repeats", unit, copies)
+    want, want_rc = _oracle(p, seq, off)
+    out, calls, how, rc = _outcome(ctx, p, seq, off, 0)
+    assert how == 1, f"left device seeding ({how})"
+    assert (rc, out) == (want_rc, want)
+    out_h, calls_h, _, rc_h = _outcome(ctx, p, seq, off, api.FORM_HOST_SEED)
+    assert (rc_h, out_h, calls_h) == (rc, out, calls)
