@@ -481,6 +481,199 @@ Estimate capacity_of(const mpoa_batch* b, int w, bool full) {
   return e;
 }
 
+int run_seeded_dev(mpoa_batch* b, const int64_t* seq_off, std::vector<mpoa_call_stat>* calls);
+int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
+               std::vector<mpoa_call_stat>* calls);
+
+// One persistent launch of k_seeded over the seeded windows `wins` (processing order): slots
+// of one workgroup per CU, each with graph pools sized for the largest window and its own
+// seeding buffers (SeedDev). Returns MPOA_OK or an error; window statuses land in d_res.
+int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vector<int>& wins,
+                float* ms_out) {
+  mpoa_ctx* ctx = b->ctx;
+  const bool keep_path = b->params.result != 0;
+  SlotCaps c = make_caps(est, b->info, wins, keep_path);
+  int64_t qtot = 1;
+  int nr = 1;
+  for (int w : wins) {
+    qtot = std::max(qtot, b->info[w].qtot);
+    nr = std::max(nr, b->info[w].n_reads);
+  }
+  const int64_t rm = qtot + (int64_t)nr * (2 * b->params.mm_w + 8);
+  const int cons_cap = c.nodes;
+  const size_t u_words = 4 * (size_t)rm + (size_t)qtot + 5 * (size_t)cons_cap +
+                         8 * (size_t)mpoa::SEED_ANCHORS;
+  // ints: rm_cnt, n_seg, n_splice, read_graph (2), splices, flag
+  const size_t i_words = (size_t)nr + 4 + 2 * (size_t)mpoa::SEED_SPLICE + 1;
+  const size_t per_slot = slot_bytes_total(c) + u_words * 8 + i_words * 4 +
+                          mpoa::SEED_SEGS * sizeof(SegDesc) + sizeof(mpoa::SeedDev);
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  const int slots = (int)std::min<size_t>(
+      std::min<size_t>(wins.size(), (size_t)std::max(1, ctx->n_cu)), free_b / 4 * 3 / per_slot);
+  if (slots < 1) {
+    g_err = "window too large for device memory";
+    return MPOA_E_OOM;
+  }
+  Bin& bin = b->rerun;  // the pools of the seeded launch
+  bin.nt = 256;
+  bin.cpl = 2;
+  bin.wins = wins;
+  int rc;
+  if ((rc = provision(bin, c, slots, free_b / 4 * 3 - slots * (per_slot - slot_bytes_total(c)))) !=
+      MPOA_OK)
+    return rc;
+  const int n_slots = bin.used_slots;
+  DevBuf<uint64_t> u_buf;
+  DevBuf<int> i_buf;
+  DevBuf<SegDesc> segs;
+  DevBuf<mpoa::SeedDev> d_seed;
+  if ((rc = u_buf.alloc(n_slots * u_words)) != MPOA_OK ||
+      (rc = i_buf.alloc(n_slots * i_words)) != MPOA_OK ||
+      (rc = segs.alloc((size_t)n_slots * mpoa::SEED_SEGS)) != MPOA_OK ||
+      (rc = d_seed.alloc(n_slots)) != MPOA_OK)
+    return rc;
+  std::vector<mpoa::SeedDev> sd(n_slots);
+  for (int s = 0; s < n_slots; s++) {
+    mpoa::SeedDev& d = sd[s];
+    uint64_t* u = u_buf.p + (size_t)s * u_words;
+    auto take_u = [&](size_t m) { uint64_t* p = u; u += m; return p; };
+    d.rmx = take_u(rm); d.rmy = take_u(rm); d.rtx = take_u(rm); d.rty = take_u(rm);
+    d.hx = take_u((size_t)qtot + cons_cap);
+    d.cmx = take_u(cons_cap); d.cmy = take_u(cons_cap);
+    d.ctx = take_u(cons_cap); d.cty = take_u(cons_cap);
+    d.ax = take_u(mpoa::SEED_ANCHORS); d.ay = take_u(mpoa::SEED_ANCHORS);
+    d.bx = take_u(mpoa::SEED_ANCHORS); d.by = take_u(mpoa::SEED_ANCHORS);
+    d.zx = take_u(mpoa::SEED_ANCHORS); d.zy = take_u(mpoa::SEED_ANCHORS);
+    d.zsx = take_u(mpoa::SEED_ANCHORS); d.zsy = take_u(mpoa::SEED_ANCHORS);
+    int* iw = i_buf.p + (size_t)s * i_words;
+    d.rm_cnt = iw; iw += nr;
+    d.n_seg = iw; iw += 1;
+    d.n_splice = iw; iw += 1;
+    d.read_graph = iw; iw += 2;
+    d.splice = iw; iw += 2 * mpoa::SEED_SPLICE;
+    d.flag = iw;
+    d.rm_off = nullptr;  // each read's minimizers at its offset in the window plus 2w + 8 each
+    d.hx_cons = qtot;
+    d.cons_cap = cons_cap;
+    d.segs = segs.p + (size_t)s * mpoa::SEED_SEGS;
+    d.dp_cap = 0;  // sub-windows run one at a time in the slot's own DP pool
+    d.k = b->params.k;
+    d.w = b->params.mm_w;
+    d.bw = 1000;  // main.cpp:65
+    d.poa_w = b->params.poa_w;
+    d.pen_gap = (float)(0.01 * d.k);  // minimizer.cpp:498
+    d.pen_skip = (float)(0.0 * 0.01 * d.k);
+  }
+  HIPCHK(hipMemcpyAsync(d_seed.p, sd.data(), n_slots * sizeof(mpoa::SeedDev),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(i_buf.p, 0, n_slots * i_words * sizeof(int), ctx->stream));
+  if ((rc = upload_order(bin, ctx->stream)) != MPOA_OK) return rc;
+  HIPCHK(hipMemsetAsync(bin.d_queue.p, 0, sizeof(int), ctx->stream));
+  KernelArgs a = args_of(b, bin);
+  a.forms = b->forms;
+  a.call_stats = nullptr;
+  a.call_off = nullptr;
+  a.seed = d_seed.p;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  if (mpoa_launch_seeded(a, n_slots, ctx->stream) != 0) {
+    g_err = "seeded launch failed";
+    return MPOA_E_DEVICE;
+  }
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = ms;
+  b->launches++;
+  b->first_slots = std::max(b->first_slots, n_slots);
+  return MPOA_OK;
+}
+
+// Window w of a seeded batch through the single-window form (its own sizing; device seeding,
+// else host seeding): for windows past a capacity of the batched form, and for every window
+// under MPOA_FORM_HOST_SEED. Its status, output and result record replace the batch's.
+int seeded_single(mpoa_batch* b, int w) {
+  mpoa_ctx* ctx = b->ctx;
+  const WinInfo& wi = b->info[w];
+  const WinDesc& wd = b->desc[w];
+  std::vector<int64_t> off(wi.n_reads + 1);
+  HIPCHK(hipMemcpy(off.data(), b->d_off.p + wd.read0, off.size() * sizeof(int64_t),
+                   hipMemcpyDeviceToHost));
+  std::string seq((size_t)(off.back() - off[0]), '\0');
+  if (!seq.empty())
+    HIPCHK(hipMemcpy(seq.data(), b->d_seq.p + off[0], seq.size(), hipMemcpyDeviceToHost));
+  for (int i = wi.n_reads; i >= 0; i--) off[i] -= off[0];
+  mpoa_params pp = b->params;
+  pp.seeding = 0;
+  pp.progressive = 0;  // the batch holds the reads in guide-tree order already
+  int32_t win_off[2] = {0, wi.n_reads};
+  mpoa_batch* nb = nullptr;
+  int rc = mpoa_batch_prepare(ctx, &pp, 1, win_off, seq.data(), off.data(), nullptr, &nb);
+  if (rc != MPOA_OK) return rc;
+  nb->params.seeding = 1;
+  nb->forms = b->forms;
+  std::vector<mpoa_call_stat> calls;
+  rc = (b->forms & MPOA_FORM_HOST_SEED) ? 1 : run_seeded_dev(nb, off.data(), &calls);
+  if (rc == 1) rc = run_seeded(nb, seq.data(), off.data(), &calls);
+  if (!nb->res.empty()) {
+    b->res[w] = nb->res[0];
+    if (nb->desc[0].out_cap > 0 &&
+        hipMemcpy(b->d_out.p + wd.out0, nb->d_out.p, (size_t)nb->desc[0].out_cap,
+                  hipMemcpyDeviceToDevice) != hipSuccess)
+      rc = MPOA_E_DEVICE;
+    if (hipMemcpy(b->d_res.p + w, &b->res[w], sizeof(WinResult), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      rc = MPOA_E_DEVICE;
+  }
+  b->launches += nb->launches;
+  mpoa_batch_free(nb);
+  return rc == MPOA_E_DEVICE || rc == MPOA_E_OOM ? rc : MPOA_OK;
+}
+
+// Seeded (-S) windows of a batch: one persistent launch (k_seeded) with capacities estimated
+// from the reads, windows that outgrew them again with worst-case graph capacities, and those
+// still past a capacity (seeding's own limits) through the single-window form.
+int execute_seeded(mpoa_batch* b, std::vector<Estimate>& est) {
+  const int n = b->n_windows;
+  b->first_slots = 0;
+  std::vector<WinResult> res(n);
+  std::vector<int> again;
+  int rc;
+  if (!(b->forms & MPOA_FORM_HOST_SEED)) {
+    float ms = 0;
+    if ((rc = seeded_pass(b, est, b->order, &ms)) != MPOA_OK) return rc;
+    b->kernel_ms = ms;
+    HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+    for (int w = 0; w < n; w++)
+      if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
+    if (!again.empty()) {
+      b->reruns += (int64_t)again.size();
+      for (int w : again) est[w] = worst_case(b->info[w]);
+      if ((rc = seeded_pass(b, est, again, &ms)) != MPOA_OK) return rc;
+      b->kernel_ms += ms;
+      HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
+      again.clear();
+      for (int w = 0; w < n; w++)
+        if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
+    }
+  } else {
+    for (int w = 0; w < n; w++) again.push_back(w);
+  }
+  b->res = res;
+  for (int w : again)
+    if ((rc = seeded_single(b, w)) != MPOA_OK) return rc;
+  b->device_ms = b->kernel_ms;
+  for (int w = 0; w < n; w++)
+    if (b->res[w].status != MPOA_OK) return b->res[w].status;
+  return MPOA_OK;
+}
+
 // The plain form: one launch per window, one workgroup, one window after another on one
 // stream, in input order. Same kernel and capacities as the batched form.
 int plain_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vector<int>& wins,
@@ -1130,10 +1323,6 @@ int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windo
   if (!ctx || !params || !out || n_windows < 0 || !win_off || !seq_off || (!seq && n_windows))
     return MPOA_E_ARG;
   *out = nullptr;
-  if (params->seeding) {
-    g_err = "seeding (-S) windows are not supported by the batch pipeline";
-    return MPOA_E_ARG;
-  }
   if (params->result < 0 || params->result > 3) {
     g_err = "result must be 0 (consensus), 1 (MSA), 2 (GFA) or 3 (MSA with a consensus row)";
     return MPOA_E_ARG;
@@ -1260,6 +1449,11 @@ int mpoa_batch_execute(mpoa_batch* b) {
   const bool keep_path = b->params.result != 0;
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
   std::vector<Estimate> est(n);
+  if (b->params.seeding) {
+    // -S sub-windows always band adaptively: rows as wide as the read
+    for (int w = 0; w < n; w++) est[w] = estimate(b->info[w], b->sc, true);
+    return execute_seeded(b, est);
+  }
   for (int w = 0; w < n; w++) est[w] = capacity_of(b, w, full);
   if (b->mode == MPOA_MODE_PLAIN) return execute_plain(b, est);
 

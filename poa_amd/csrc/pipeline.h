@@ -141,6 +141,8 @@ struct CallStat {
   int nodes, edges;              // graph size when the call ran
 };
 
+struct SeedDev;
+
 struct KernelArgs {
   DpScores scores;
   SlotCaps caps;
@@ -159,6 +161,7 @@ struct KernelArgs {
   CallStat* call_stats;     // optional: per DP call, window w's calls from call_off[w]
   const int64_t* call_off;  // n_windows + 1 offsets (reads - 1 per window)
   int forms;                // MPOA_FORM_* bits: fallback forms taken for every read (0 in use)
+  const SeedDev* seed;      // seeded batches: each slot's seeding buffers (k_seeded)
 };
 
 // -S sub-window of the current read: the DP of read characters [q0, q0 + qlen) against the
@@ -299,6 +302,9 @@ int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t s
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream);
 // Device seeding: every read's minimizers (one workgroup per read), before the first commit.
 int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream);
+// Seeded (-S) windows of a batch: the persistent launch, one window per workgroup at a time
+// (256 threads, one workgroup per CU), every read seeded and its sub-windows run in turn.
+int mpoa_launch_seeded(const mpoa::KernelArgs& a, int n_slots, hipStream_t stream);
 // Device address of this device's allocation-free scratch array (MPOA_NOALLOC_SCRATCH_BYTES),
 // looked up once per device with that device current. 0, or -1 when the lookup fails.
 int mpoa_scratch_address(int device, void** out);

@@ -2721,7 +2721,8 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
     s.dev.read_graph[2 * r + 1] = S.n_in_e;
   }
   if (s.use_dev && S.status == MPOA_OK && r + 1 < s.n_reads) {
-    seed_next<NT>(s, A, S, reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
+    seed_next<NT>(a, wd, s.dev, r + 1, r + 1, A, S, reinterpret_cast<uint8_t*>(lds4),
+                  (int)sizeof(lds4));
   } else if (S.status == MPOA_OK && r + 1 < s.n_reads) {
     if (S.cons_len > s.cons_cap) {
       if (tid == 0) S.status = MPOA_E_OVERFLOW;
@@ -2737,6 +2738,104 @@ __global__ __launch_bounds__(NT) void k_commit(SeedArgs s) {
   if (r + 1 == s.n_reads || S.status != MPOA_OK)
     window_outputs<NT>(a, A, S, 0, reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
   if (tid == 0) state_store(S, *s.state);
+}
+
+// A seeded (-S) window of a batch in one workgroup (minipoa's loop, main.cpp:110-121, with
+// alignment()'s -S branch, align.cpp:1752-1797): the reads' minimizers, then read by read its
+// sub-windows one after another (each a poa() call on a rank window of the graph, writing its
+// stretch of the read's res), the anchor bases between them, the commit, and the seeding of
+// the next read against the new consensus. lds: the workgroup's LDS, shared by the DP ring,
+// the sort and the seeding, which never run at once.
+template <int NT, int CPL, class RingT>
+__device__ void run_window_seeded(const KernelArgs& a, Arena& A, WinShared& S, RingT& R,
+                                  uint8_t* lds, int lds_bytes, const SeedDev& d, int w) {
+  const WinDesc& wd = a.win[w];
+  const int tid = threadIdx.x;
+  const bool keep_path = a.scores.result != 0;
+  window_begin<NT>(a, A, S, w);
+  int* sh = reinterpret_cast<int*>(lds + lds_bytes - 64);
+  bool ok = true;
+  for (int r = 0; r < wd.n_reads && ok; r++)
+    ok = seed_read_minimizers<NT>(a, wd, d, r, lds, lds_bytes - 64, sh);
+  if (tid == 0) {
+    if (!ok) S.status = MPOA_E_OVERFLOW;
+    d.n_seg[0] = 0;
+    d.n_splice[0] = 0;
+  }
+  __syncthreads();
+  int path0 = 0;
+  for (int r = 0; r < wd.n_reads && S.status == MPOA_OK; r++) {
+    const int64_t q0 = a.read_off[wd.read0 + r], q1 = a.read_off[wd.read0 + r + 1];
+    const int qlen = (int)(q1 - q0), m = qlen + 1;
+    if (qlen > a.caps.qmax || (keep_path && path0 + qlen > a.caps.path)) {
+      if (tid == 0) S.status = MPOA_E_OVERFLOW;
+      __syncthreads();
+      break;
+    }
+    if (tid == 0) S.read = r;
+    const int n_seg = d.n_seg[0], n_splice = d.n_splice[0];
+    if (n_seg == 0) {  // a graph without bases takes the read as a new path
+      load_query<NT>(A, S, R, a.read_seq + q0, qlen);
+      trivial_res<NT>(A, m);
+    } else {
+      const bool adaptive = a.scores.f > 0;  // -S always bands adaptively (align.cpp:1780)
+      for (int g = 0; g < n_seg && S.status == MPOA_OK; g++) {
+        const SegDesc sd = d.segs[g];
+        Arena As = A;
+        As.res_from += sd.res0;
+        As.res_aligned += sd.res0;
+        As.res_base += sd.res0;
+        if (tid == 0) {
+          S.beg_i = A.rank[sd.beg_id];
+          S.end_i = A.rank[sd.end_id];
+          S.n_rows = S.end_i - S.beg_i + 1;
+        }
+        __syncthreads();
+        load_query<NT>(As, S, R, a.read_seq + q0 + sd.q0, sd.qlen);
+        if (S.q_lds) poa_call<NT, CPL, true, false, RingT>(a, As, S, R, adaptive);
+        else poa_call<NT, CPL, false, false, RingT>(a, As, S, R, adaptive);
+        __syncthreads();
+      }
+      if (S.status == MPOA_OK) {
+        // anchor bases between the sub-windows: existing consensus nodes (align.cpp:1794-1797)
+        for (int k = tid; k < n_splice; k += NT) {
+          const int pos = d.splice[2 * k], id = d.splice[2 * k + 1];
+          A.res_from[pos] = id;
+          A.res_base[pos] = A.base[id];
+          A.res_aligned[pos] = -1;
+        }
+      }
+      __syncthreads();
+    }
+    if (S.status != MPOA_OK) break;
+    commit_read<NT>(a, A, S, r, m, path0, r + 1 == wd.n_reads, true, lds, lds_bytes);
+    path0 += qlen;
+    __syncthreads();
+    if (S.status != MPOA_OK) break;
+    if (r + 1 < wd.n_reads) seed_next<NT>(a, wd, d, r + 1, 0, A, S, lds, lds_bytes);
+  }
+  window_outputs<NT>(a, A, S, w, lds, lds_bytes);
+}
+
+constexpr int SEEDED_LDS = 156 * 1024;
+
+template <int NT, int CPL>
+__global__ __launch_bounds__(NT) void k_seeded(KernelArgs a) {
+  __shared__ WinShared S;
+  __shared__ __align__(16) uint8_t lds[SEEDED_LDS];
+  using RingT = Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP>;
+  static_assert(sizeof(RingT) <= SEEDED_LDS, "the DP ring shares the seeding LDS");
+  RingT& R = *reinterpret_cast<RingT*>(lds);
+  Arena A = carve(a, blockIdx.x);
+  const SeedDev d = a.seed[blockIdx.x];
+  for (;;) {
+    if (threadIdx.x == 0) S.job = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int job = S.job;
+    __syncthreads();
+    if (job >= a.n_jobs) break;
+    run_window_seeded<NT, CPL>(a, A, S, R, lds, SEEDED_LDS, d, a.order[job]);
+  }
 }
 
 // The call-level operations on a graph kept in device memory (GraphOpArgs): graph::init, one
@@ -3059,6 +3158,13 @@ int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream) {
   using namespace mpoa;
   if (s.n_reads <= 0) return 0;
   hipLaunchKernelGGL((k_seed_reads<256>), dim3(s.n_reads), dim3(256), 0, stream, s);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mpoa_launch_seeded(const mpoa::KernelArgs& a, int n_slots, hipStream_t stream) {
+  using namespace mpoa;
+  if (n_slots <= 0) return 0;
+  hipLaunchKernelGGL((k_seeded<256, 2>), dim3(n_slots), dim3(256), 0, stream, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -192,30 +192,45 @@ __device__ __forceinline__ int seed_bound(const uint64_t* x, int n, uint64_t v, 
   return lo;
 }
 
-// Minimizers of read r (one workgroup per read), sorted by x.
+// Where read r's minimizers start in the window's minimizer arrays (capacity qlen + 2w + 8 per
+// read): the host's offsets when it gave them, else the read's offset in the window plus
+// 2w + 8 per read before it.
+__device__ __forceinline__ int64_t seed_rm0(const SeedDev& d, const KernelArgs& a,
+                                            const WinDesc& wd, int r) {
+  if (d.rm_off) return d.rm_off[r];
+  return (a.read_off[wd.read0 + r] - a.read_off[wd.read0]) + (int64_t)r * (2 * d.w + 8);
+}
+
+// Minimizers of read r of window wd, sorted by x (the whole workgroup). False when they
+// outgrow their capacity or the LDS.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_seed_reads(SeedArgs s) {
-  __shared__ uint64_t lds8[(SEED_SORT * 12 + SEED_SORT_AUX) / 8];
-  __shared__ int sh[NT / 64];
-  const SeedDev& d = s.dev;
-  const KernelArgs& a = s.graph;
-  const int r = blockIdx.x;
-  const WinDesc& wd = a.win[0];
+__device__ bool seed_read_minimizers(const KernelArgs& a, const WinDesc& wd, const SeedDev& d,
+                                     int r, uint8_t* lds, int lds_bytes, int* sh) {
   const int64_t q0 = a.read_off[wd.read0 + r], base0 = a.read_off[wd.read0];
   const int len = (int)(a.read_off[wd.read0 + r + 1] - q0);
   const uint8_t* seq = a.read_seq + q0;
   uint64_t* hx = d.hx + (q0 - base0);
   seed_positions<NT>([&](int i) { return code_of(seq[i]); }, len, d.k, hx);
-  const int64_t m0 = d.rm_off[r];
-  const int cap = (int)(d.rm_off[r + 1] - m0);
+  const int64_t m0 = seed_rm0(d, a, wd, r);
+  const int cap = len + 2 * d.w + 8;
   const int n = seed_emit<NT>(hx, len, d.w, (uint32_t)r, d.rtx + m0, d.rty + m0, cap, sh);
   bool ok = n >= 0;
-  if (ok) ok = seed_sort<NT>(d.rtx + m0, d.rty + m0, d.rmx + m0, d.rmy + m0, n,
-                             reinterpret_cast<uint8_t*>(lds8), (int)sizeof(lds8));
+  if (ok) ok = seed_sort<NT>(d.rtx + m0, d.rty + m0, d.rmx + m0, d.rmy + m0, n, lds, lds_bytes);
   if (threadIdx.x == 0) {
     d.rm_cnt[r] = ok ? n : 0;
     if (!ok) atomicOr(d.flag, SEED_WHY_READS);
   }
+  __syncthreads();
+  return ok;
+}
+
+// Minimizers of read r (one workgroup per read), sorted by x.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_seed_reads(SeedArgs s) {
+  __shared__ uint64_t lds8[(SEED_SORT * 12 + SEED_SORT_AUX) / 8];
+  __shared__ int sh[NT / 64];
+  seed_read_minimizers<NT>(s.graph, s.graph.win[0], s.dev, blockIdx.x,
+                           reinterpret_cast<uint8_t*>(lds8), (int)sizeof(lds8), sh);
 }
 
 // The pieces of the local chains kept in LDS (SEED_ANCHORS ints each).
@@ -223,17 +238,15 @@ struct ChainLds {
   int *tp, *qp, *f, *p, *t, *v;
 };
 
-// Seeds read r1 against the consensus the commit just built (A.cons_*, S.cons_len) and writes
-// its sub-windows (alignment()'s -S branch, align.cpp:1752-1797). lds: lds_bytes of free LDS.
-// Returns false when a capacity is exceeded (*flag set, S.status MPOA_E_OVERFLOW).
+// Seeds read r1 of window wd against the consensus the commit of read r1 - 1 just built
+// (A.cons_*, S.cons_len) and writes its sub-windows (alignment()'s -S branch,
+// align.cpp:1752-1797) at index ri of the per-read outputs (n_seg, n_splice, segs, splice).
+// lds: lds_bytes of free LDS. Returns false when a capacity is exceeded (*flag set, S.status
+// MPOA_E_OVERFLOW) or the read ends the window (MPOA_E_EMPTY_WINDOW).
 template <int NT>
-__device__ bool seed_next(const SeedArgs& s, Arena& A, WinShared& S, uint8_t* lds,
-                          int lds_bytes) {
-  const SeedDev& d = s.dev;
-  const KernelArgs& a = s.graph;
+__device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev& d, int r1,
+                          int ri, Arena& A, WinShared& S, uint8_t* lds, int lds_bytes) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r1 = s.read + 1;
-  const WinDesc& wd = a.win[0];
   const int64_t rq0 = a.read_off[wd.read0 + r1];
   const int qlen = (int)(a.read_off[wd.read0 + r1 + 1] - rq0);
   int* sh = reinterpret_cast<int*>(lds + lds_bytes - 64);
@@ -244,14 +257,14 @@ __device__ bool seed_next(const SeedArgs& s, Arena& A, WinShared& S, uint8_t* ld
     if (tid == 0) {
       atomicOr(d.flag, why);
       S.status = MPOA_E_OVERFLOW;
-      d.n_seg[r1] = 0;
-      d.n_splice[r1] = 0;
+      d.n_seg[ri] = 0;
+      d.n_splice[ri] = 0;
     }
     __syncthreads();
     return false;
   };
   if (S.n_nodes <= 2) {  // alignment(): a graph without bases takes the read as a new path
-    if (tid == 0) { d.n_seg[r1] = 0; d.n_splice[r1] = 0; }
+    if (tid == 0) { d.n_seg[ri] = 0; d.n_splice[ri] = 0; }
     __syncthreads();
     return true;
   }
@@ -260,14 +273,14 @@ __device__ bool seed_next(const SeedArgs& s, Arena& A, WinShared& S, uint8_t* ld
   // Consensus minimizers (collect_anchors_bycons, minimizer.cpp:541-551), sorted by x.
   uint64_t* hx = d.hx + d.hx_cons;
   seed_positions<NT>([&](int i) { return code_of(A.cons_ch[i]); }, L, k, hx);
-  const int n_c = seed_emit<NT>(hx, L, d.w, (uint32_t)s.n_reads, d.ctx, d.cty, d.cons_cap, sh);
+  const int n_c = seed_emit<NT>(hx, L, d.w, (uint32_t)wd.n_reads, d.ctx, d.cty, d.cons_cap, sh);
   if (n_c < 0) return fail(SEED_WHY_CONS_MM);
   if (!seed_sort<NT>(d.ctx, d.cty, d.cmx, d.cmy, n_c, lds, lds_bytes))
     return fail(SEED_WHY_CONS_SORT);
   // Anchors (collect_anchors, minimizer.cpp:404-442): for each consensus minimizer in order,
   // every read minimizer with the same x in order; x = rid << 32 | tpos, y = span << 32 | qpos.
-  const uint64_t* qx = d.rmx + d.rm_off[r1];
-  const uint64_t* qy = d.rmy + d.rm_off[r1];
+  const uint64_t* qx = d.rmx + seed_rm0(d, a, wd, r1);
+  const uint64_t* qy = d.rmy + seed_rm0(d, a, wd, r1);
   const int n_q = d.rm_cnt[r1];
   int n_a = 0;
   for (int c0 = 0; c0 < n_c; c0 += NT) {
@@ -435,8 +448,8 @@ __device__ bool seed_next(const SeedArgs& s, Arena& A, WinShared& S, uint8_t* ld
   int* cs = C.tp;
   int* pc = C.qp;
   int* pick = C.v;  // picked anchors, back to front: (tpos, qpos) pairs, SEED_SEGS kept
-  SegDesc* segs = d.segs + (int64_t)r1 * SEED_SEGS;
-  int* splice = d.splice + (int64_t)r1 * 2 * SEED_SPLICE;
+  SegDesc* segs = d.segs + (int64_t)ri * SEED_SEGS;
+  int* splice = d.splice + (int64_t)ri * 2 * SEED_SPLICE;
   if (tid == 0) {
     int n_pick = 0;
     auto take = [&](int ct, int cq) {
@@ -580,8 +593,8 @@ __device__ bool seed_next(const SeedArgs& s, Arena& A, WinShared& S, uint8_t* ld
       S.status = status;
       if (status == MPOA_E_OVERFLOW) atomicOr(d.flag, why);
     }
-    d.n_seg[r1] = n_seg;
-    d.n_splice[r1] = n_sp;
+    d.n_seg[ri] = n_seg;
+    d.n_splice[ri] = n_sp;
   }
   __syncthreads();
   return S.status == MPOA_OK;

@@ -128,3 +128,39 @@ def test_repeats_against_oracle(ctx, unit, copies):
     assert (rc, out) == (want_rc, want)
     out_h, calls_h, _, rc_h = _outcome(ctx, p, seq, off, api.FORM_HOST_SEED)
     assert (rc_h, out_h, calls_h) == (rc, out, calls)
+
+
+def _finishing_windows(p, seed: int, n: int):
+    """A seeded synthetic batch restricted to the windows minipoa finishes, with their texts."""
+    b = synth.config3(seed=seed, n_windows=n)
+    keep, want = [], []
+    for w in range(n):
+        r0, r1 = int(b.win_off[w]), int(b.win_off[w + 1])
+        o = b.seq_off[r0:r1 + 1]
+        text, rc = _oracle(p, b.seq[o[0]:o[-1]].copy(), (o - o[0]).astype(np.int64))
+        if rc == 0:
+            keep.append(w)
+            want.append(text)
+    return b.subset(keep), want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flags", [dict(seeding=1, result=0), dict(seeding=1, result=1),
+                                   dict(seeding=1, k=15, mm_w=5, result=3),
+                                   dict(seeding=1, progressive=1, result=1)],
+                         ids=["S_r0", "S_r1", "S_k15w5_r3", "S_p_r1"])
+def test_seeded_batch_against_oracle(ctx, flags):
+    """-S windows in the batch API: one workgroup per window seeds every read itself and runs
+    its sub-windows in turn (k_seeded); the host-seeded form gives the same texts."""
+    p = api.Params(**flags)
+    sub, want = _finishing_windows(p, 77 + flags.get("k", 19), 48)
+    assert len(want) >= 30
+    _say("seeded batch", flags, len(want), "windows")
+    bat = api.Batch(ctx, p, sub.win_off, sub.seq, sub.seq_off)
+    bat.execute()
+    got = bat.results()
+    assert got.texts == want
+    bat.set_forms(api.FORM_HOST_SEED)
+    bat.execute()
+    assert bat.results().texts == want
+    bat.free()
