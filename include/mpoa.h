@@ -276,7 +276,11 @@ typedef struct mpoa_batch_stats {
 /* Uploads window w = reads [win_off[w], win_off[w+1]) of seq/seq_off, sizes the per-window
  * arenas and allocates device memory. size_hint (optional, n_windows entries, may be NULL)
  * are predicted DP cells per window, used to order the work queue and to pre-size the DP
- * arenas; any misprediction is caught on the device and that window re-run exactly. */
+ * arenas; any misprediction is caught on the device and that window re-run exactly.
+ * With params->seeding (-S) every window runs in one workgroup of a persistent launch that
+ * seeds each read itself (minimizers, anchors, chains on the consensus so far) and runs the
+ * read's sub-windows in turn; tile, schedule, graph replay and call traces do not apply, and
+ * a window past a capacity of that launch runs alone as mpoa_align_reads would run it. */
 int mpoa_batch_prepare(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_windows,
                        const int32_t* win_off, const char* seq, const int64_t* seq_off,
                        const double* size_hint, mpoa_batch** out);

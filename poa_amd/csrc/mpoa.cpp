@@ -557,7 +557,9 @@ int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vect
     d.hx_cons = qtot;
     d.cons_cap = cons_cap;
     d.segs = segs.p + (size_t)s * mpoa::SEED_SEGS;
-    d.dp_cap = 0;  // sub-windows run one at a time in the slot's own DP pool
+    // sub-windows run one at a time in the slot's own DP pool, each checked against it by
+    // the DP itself (a window past it re-runs with worst-case capacities)
+    d.dp_cap = INT64_MAX;
     d.k = b->params.k;
     d.w = b->params.mm_w;
     d.bw = 1000;  // main.cpp:65

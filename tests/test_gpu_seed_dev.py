@@ -160,6 +160,8 @@ def test_seeded_batch_against_oracle(ctx, flags):
     bat.execute()
     got = bat.results()
     assert got.texts == want
+    assert got.stats.reruns == 0, "windows left the seeded launch"
+    assert got.stats.launches == 1
     bat.set_forms(api.FORM_HOST_SEED)
     bat.execute()
     assert bat.results().texts == want
