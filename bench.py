@@ -172,6 +172,57 @@ def single_window(ctx, cores: int, with_cpu: bool, log) -> dict:
     return out
 
 
+def seeded_batch(ctx, batch, with_cpu: bool, cpu_sample: int, log) -> dict:
+    """The same windows under -S through the batch API (k_seeded: one workgroup per window
+    seeds every read itself and runs its sub-windows in turn), best of two executes; the
+    oracle on the first cpu_sample windows on one host core, outputs compared."""
+    import collections
+    from poa_amd import api
+    params = api.Params(seeding=1, result=0)
+    n = len(batch.win_off) - 1
+    b = api.Batch(ctx, params, batch.win_off, batch.seq, batch.seq_off)
+    best = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        try:
+            b.execute()
+        except api.MpoaError:
+            pass  # windows minipoa does not finish report their own status
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    ws, _ = b.window_stats()
+    status = [int(x.status) for x in ws]
+    texts = b.results().texts
+    st = b.stats()
+    b.free()
+    rec = {"flags": "-S -r 0", "windows": n, "windows_per_s": round(n / best, 1),
+           "kernel_ms": round(st.kernel_ms, 1), "launches": int(st.launches),
+           "reruns": int(st.reruns),
+           "statuses": {str(k): v for k, v in sorted(collections.Counter(status).items())}}
+    if with_cpu and cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        m = min(cpu_sample, n)
+        same = diverged = 0
+        t0 = time.perf_counter()
+        for w in range(m):
+            win = batch.window(w)
+            try:
+                want, rc = oracle_lib.window(params, win.seq, win.off)[0], 0
+            except RuntimeError as e:
+                want, rc = None, int(str(e).split()[-1])
+            if rc == status[w] and (rc != 0 or want == texts[w]):
+                same += 1
+            else:
+                diverged += 1
+        cw = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(m / cw, 2), "unit": "windows/s", "cores": 1,
+                               "kind": "port", "sample": f"first {m} windows, -S -r 0",
+                               "windows_identical": same, "windows_differing": diverged}
+    log(f"seeded batch: {rec['windows_per_s']} windows/s, statuses {rec['statuses']}")
+    return rec
+
+
 def ingest(batch, threads: int, log) -> dict:
     """Read-set ingestion as a caller with files would do it: every window of the workload
     written as its own FASTA file (60-column lines), then all files parsed and packed by the
@@ -307,6 +358,8 @@ def main() -> int:
                     help="skip the single-window configurations")
     ap.add_argument("--no-ingest", action="store_true",
                     help="skip the timing of read-set ingestion from FASTA files")
+    ap.add_argument("--no-seeded", action="store_true",
+                    help="skip the -S run of the same windows through the batch API")
     ap.add_argument("--no-ab", action="store_true",
                     help="skip the learned-against-estimate scheduling A/B")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -515,6 +568,8 @@ def main() -> int:
         line["ingest"] = ingest(batch, cores, log)
     if not args.no_single:
         line["single_window"] = single_window(ctx, cores, not args.no_cpu, log)
+    if not args.no_seeded:
+        line["seeded_batch"] = seeded_batch(ctx, batch, not args.no_cpu, 200, log)
     ctx.close()
     print(json.dumps(line), flush=True)
     return 0
