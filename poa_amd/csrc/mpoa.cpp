@@ -485,13 +485,30 @@ int run_seeded_dev(mpoa_batch* b, const int64_t* seq_off, std::vector<mpoa_call_
 int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
                std::vector<mpoa_call_stat>* calls);
 
-// One persistent launch of k_seeded over the seeded windows `wins` (processing order): slots
-// of one workgroup per CU, each with graph pools sized for the largest window and its own
-// seeding buffers (SeedDev). Returns MPOA_OK or an error; window statuses land in d_res.
-int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vector<int>& wins,
-                float* ms_out) {
+// A persistent launch of k_seeded in flight: its seeding buffers and events.
+struct SeededLaunch {
+  Bin* bin = nullptr;
+  int nt = 0, n_slots = 0;
+  DevBuf<uint64_t> u_buf;
+  DevBuf<int> i_buf;
+  DevBuf<SegDesc> segs;
+  DevBuf<mpoa::SeedDev> d_seed;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~SeededLaunch() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+// Enqueues one persistent launch of k_seeded (nt threads per window) over the seeded windows
+// of `bin` (processing order) on `stream`: as many slots as the tile keeps resident and
+// `budget` bytes allow, each with graph pools sized for the bin's largest window and its own
+// seeding buffers (SeedDev). Window statuses land in d_res once it completes.
+int seeded_enqueue(mpoa_batch* b, const std::vector<Estimate>& est, Bin& bin, int nt,
+                   size_t budget, hipStream_t stream, SeededLaunch& L) {
   mpoa_ctx* ctx = b->ctx;
   const bool keep_path = b->params.result != 0;
+  const std::vector<int>& wins = bin.wins;
   SlotCaps c = make_caps(est, b->info, wins, keep_path);
   int64_t qtot = 1;
   int nr = 1;
@@ -505,38 +522,52 @@ int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vect
                          8 * (size_t)mpoa::SEED_ANCHORS;
   // ints: rm_cnt, n_seg, n_splice, read_graph (2), splices, flag
   const size_t i_words = (size_t)nr + 4 + 2 * (size_t)mpoa::SEED_SPLICE + 1;
-  const size_t per_slot = slot_bytes_total(c) + u_words * 8 + i_words * 4 +
-                          mpoa::SEED_SEGS * sizeof(SegDesc) + sizeof(mpoa::SeedDev);
-  size_t free_b = 0, total_b = 0;
-  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  const size_t extra = u_words * 8 + i_words * 4 + mpoa::SEED_SEGS * sizeof(SegDesc) +
+                       sizeof(mpoa::SeedDev);
+  const size_t per_slot = slot_bytes_total(c) + extra;
+  const int per_cu = std::max(1, mpoa_seeded_occupancy(nt));
   const int slots = (int)std::min<size_t>(
-      std::min<size_t>(wins.size(), (size_t)std::max(1, ctx->n_cu)), free_b / 4 * 3 / per_slot);
+      std::min<size_t>(wins.size(), (size_t)std::max(1, ctx->n_cu) * per_cu), budget / per_slot);
   if (slots < 1) {
     g_err = "window too large for device memory";
     return MPOA_E_OOM;
   }
-  Bin& bin = b->rerun;  // the pools of the seeded launch
-  bin.nt = 256;
-  bin.cpl = 2;
-  bin.wins = wins;
   int rc;
-  if ((rc = provision(bin, c, slots, free_b / 4 * 3 - slots * (per_slot - slot_bytes_total(c)))) !=
-      MPOA_OK)
-    return rc;
+  const int want = (int)std::min<size_t>(wins.size(), (size_t)std::max(1, ctx->n_cu) * per_cu);
+  const SlotCaps& h = bin.caps;
+  if (bin.d_dp.p && bin.nt == nt && h.nodes >= c.nodes && h.edges >= c.edges &&
+      h.qmax >= c.qmax && h.path >= c.path && h.dp >= c.dp &&
+      (bin.n_slots >= want || bin.n_slots >= slots)) {
+    // the pools of an earlier execute cover this one: keep them (re-allocating costs seconds)
+    c = h;
+    bin.used_slots = std::min(bin.n_slots, want);
+  } else {
+    // The DP pool takes what the budget leaves, up to full-width rows over the whole graph,
+    // so that few sub-windows outgrow it (each would re-run its window).
+    int64_t full = c.dp;
+    for (int w : wins) full = std::max(full, estimate(b->info[w], b->sc, true).dp);
+    const int64_t spare = (int64_t)((budget - (size_t)slots * per_slot) / (size_t)slots / 12);
+    if (full > c.dp && spare > 0) {
+      c.dp += std::min(full - c.dp, spare) / 64 * 64;
+      mpoa::finish_caps(c);
+    }
+    bin.nt = nt;
+    bin.cpl = nt == 64 ? 4 : 2;
+    if ((rc = provision(bin, c, slots, budget - (size_t)slots * extra)) != MPOA_OK) return rc;
+  }
   const int n_slots = bin.used_slots;
-  DevBuf<uint64_t> u_buf;
-  DevBuf<int> i_buf;
-  DevBuf<SegDesc> segs;
-  DevBuf<mpoa::SeedDev> d_seed;
-  if ((rc = u_buf.alloc(n_slots * u_words)) != MPOA_OK ||
-      (rc = i_buf.alloc(n_slots * i_words)) != MPOA_OK ||
-      (rc = segs.alloc((size_t)n_slots * mpoa::SEED_SEGS)) != MPOA_OK ||
-      (rc = d_seed.alloc(n_slots)) != MPOA_OK)
+  L.bin = &bin;
+  L.nt = nt;
+  L.n_slots = n_slots;
+  if ((rc = L.u_buf.alloc(n_slots * u_words)) != MPOA_OK ||
+      (rc = L.i_buf.alloc(n_slots * i_words)) != MPOA_OK ||
+      (rc = L.segs.alloc((size_t)n_slots * mpoa::SEED_SEGS)) != MPOA_OK ||
+      (rc = L.d_seed.alloc(n_slots)) != MPOA_OK)
     return rc;
   std::vector<mpoa::SeedDev> sd(n_slots);
   for (int s = 0; s < n_slots; s++) {
     mpoa::SeedDev& d = sd[s];
-    uint64_t* u = u_buf.p + (size_t)s * u_words;
+    uint64_t* u = L.u_buf.p + (size_t)s * u_words;
     auto take_u = [&](size_t m) { uint64_t* p = u; u += m; return p; };
     d.rmx = take_u(rm); d.rmy = take_u(rm); d.rtx = take_u(rm); d.rty = take_u(rm);
     d.hx = take_u((size_t)qtot + cons_cap);
@@ -546,7 +577,7 @@ int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vect
     d.bx = take_u(mpoa::SEED_ANCHORS); d.by = take_u(mpoa::SEED_ANCHORS);
     d.zx = take_u(mpoa::SEED_ANCHORS); d.zy = take_u(mpoa::SEED_ANCHORS);
     d.zsx = take_u(mpoa::SEED_ANCHORS); d.zsy = take_u(mpoa::SEED_ANCHORS);
-    int* iw = i_buf.p + (size_t)s * i_words;
+    int* iw = L.i_buf.p + (size_t)s * i_words;
     d.rm_cnt = iw; iw += nr;
     d.n_seg = iw; iw += 1;
     d.n_splice = iw; iw += 1;
@@ -556,7 +587,7 @@ int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vect
     d.rm_off = nullptr;  // each read's minimizers at its offset in the window plus 2w + 8 each
     d.hx_cons = qtot;
     d.cons_cap = cons_cap;
-    d.segs = segs.p + (size_t)s * mpoa::SEED_SEGS;
+    d.segs = L.segs.p + (size_t)s * mpoa::SEED_SEGS;
     // sub-windows run one at a time in the slot's own DP pool, each checked against it by
     // the DP itself (a window past it re-runs with worst-case capacities)
     d.dp_cap = INT64_MAX;
@@ -567,33 +598,35 @@ int seeded_pass(mpoa_batch* b, const std::vector<Estimate>& est, const std::vect
     d.pen_gap = (float)(0.01 * d.k);  // minimizer.cpp:498
     d.pen_skip = (float)(0.0 * 0.01 * d.k);
   }
-  HIPCHK(hipMemcpyAsync(d_seed.p, sd.data(), n_slots * sizeof(mpoa::SeedDev),
-                        hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemsetAsync(i_buf.p, 0, n_slots * i_words * sizeof(int), ctx->stream));
-  if ((rc = upload_order(bin, ctx->stream)) != MPOA_OK) return rc;
-  HIPCHK(hipMemsetAsync(bin.d_queue.p, 0, sizeof(int), ctx->stream));
+  HIPCHK(hipMemcpyAsync(L.d_seed.p, sd.data(), n_slots * sizeof(mpoa::SeedDev),
+                        hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemsetAsync(L.i_buf.p, 0, n_slots * i_words * sizeof(int), stream));
+  if ((rc = upload_order(bin, stream)) != MPOA_OK) return rc;
+  HIPCHK(hipMemsetAsync(bin.d_queue.p, 0, sizeof(int), stream));
   KernelArgs a = args_of(b, bin);
   a.forms = b->forms;
   a.call_stats = nullptr;
   a.call_off = nullptr;
-  a.seed = d_seed.p;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, ctx->stream));
-  if (mpoa_launch_seeded(a, n_slots, ctx->stream) != 0) {
+  a.seed = L.d_seed.p;
+  HIPCHK(hipEventCreate(&L.e0));
+  HIPCHK(hipEventCreate(&L.e1));
+  HIPCHK(hipEventRecord(L.e0, stream));
+  if (mpoa_launch_seeded(a, n_slots, nt, stream) != 0) {
     g_err = "seeded launch failed";
     return MPOA_E_DEVICE;
   }
-  HIPCHK(hipEventRecord(e1, ctx->stream));
-  HIPCHK(hipEventSynchronize(e1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *ms_out = ms;
+  HIPCHK(hipEventRecord(L.e1, stream));
   b->launches++;
-  b->first_slots = std::max(b->first_slots, n_slots);
+  b->first_slots += n_slots;
+  return MPOA_OK;
+}
+
+// Waits for an enqueued seeded launch; its HIP-event milliseconds.
+int seeded_wait(SeededLaunch& L, float* ms) {
+  *ms = 0;
+  if (!L.e1) return MPOA_OK;
+  HIPCHK(hipEventSynchronize(L.e1));
+  HIPCHK(hipEventElapsedTime(ms, L.e0, L.e1));
   return MPOA_OK;
 }
 
@@ -638,26 +671,73 @@ int seeded_single(mpoa_batch* b, int w) {
   return rc == MPOA_E_DEVICE || rc == MPOA_E_OOM ? rc : MPOA_OK;
 }
 
-// Seeded (-S) windows of a batch: one persistent launch (k_seeded) with capacities estimated
-// from the reads, windows that outgrew them again with worst-case graph capacities, and those
-// still past a capacity (seeding's own limits) through the single-window form.
+// Capacities of a seeded window: graph as for any window; DP for the largest sub-window,
+// which the seeding keeps to about min_w = consensus / 24 of the read between anchors, taken
+// four times over, on rows of up to the whole graph (a window past it re-runs).
+Estimate seeded_estimate(const mpoa_batch* b, int w) {
+  const WinInfo& wi = b->info[w];
+  Estimate e = estimate(wi, b->sc, true);
+  const int64_t width = std::min<int64_t>((int64_t)(wi.qmax + 1) / 8 * 8 + 16,
+                                          (int64_t)(wi.qmax / 6 + 2) / 8 * 8 + 64);
+  e.dp = (int64_t)e.nodes * width;
+  return e;
+}
+
+// Seeded (-S) windows of a batch, split like the non-seeded launches: windows with reads up
+// to 1,500 bases on the 64-thread seeded tile (30 KB of LDS, several per CU), the rest on the
+// 256-thread tile (one per CU), both launches at once on two streams. Windows that outgrow
+// their estimate (graph, or a seeding past the small tile's LDS) run again on the 256-thread
+// tile with worst-case graph capacities; those still past a capacity (seeding's own limits)
+// run through the single-window form.
 int execute_seeded(mpoa_batch* b, std::vector<Estimate>& est) {
+  mpoa_ctx* ctx = b->ctx;
   const int n = b->n_windows;
   b->first_slots = 0;
   std::vector<WinResult> res(n);
   std::vector<int> again;
   int rc;
   if (!(b->forms & MPOA_FORM_HOST_SEED)) {
-    float ms = 0;
-    if ((rc = seeded_pass(b, est, b->order, &ms)) != MPOA_OK) return rc;
-    b->kernel_ms = ms;
+    for (int w = 0; w < n; w++) est[w] = seeded_estimate(b, w);
+    b->small.wins.clear();
+    b->wide.wins.clear();
+    for (int w : b->order) (b->info[w].qmax <= 1500 ? b->small : b->wide).wins.push_back(w);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    // what the bins hold already counts as free for them (provision reuses or replaces it)
+    auto held = [](const Bin& x) {
+      return x.d_int.n * sizeof(int) + x.d_byte.n + x.d_dp.n * sizeof(int);
+    };
+    const size_t avail = free_b + held(b->small) + held(b->wide);
+    SeededLaunch ls, lw;
+    hipEvent_t ev0;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventRecord(ev0, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream3, ev0, 0));
+    (void)hipEventDestroy(ev0);
+    if (!b->wide.wins.empty() &&
+        (rc = seeded_enqueue(b, est, b->wide, 256, avail / 4, ctx->stream3, lw)) != MPOA_OK)
+      return rc;
+    if (!b->small.wins.empty() &&
+        (rc = seeded_enqueue(b, est, b->small, 64, avail / 2, ctx->stream, ls)) != MPOA_OK)
+      return rc;
+    float ms_s = 0, ms_w = 0;
+    if ((rc = seeded_wait(ls, &ms_s)) != MPOA_OK || (rc = seeded_wait(lw, &ms_w)) != MPOA_OK)
+      return rc;
+    b->kernel_ms = std::max(ms_s, ms_w);
     HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
     for (int w = 0; w < n; w++)
       if (res[w].status == MPOA_E_OVERFLOW) again.push_back(w);
     if (!again.empty()) {
       b->reruns += (int64_t)again.size();
       for (int w : again) est[w] = worst_case(b->info[w]);
-      if ((rc = seeded_pass(b, est, again, &ms)) != MPOA_OK) return rc;
+      b->rerun.wins = again;
+      HIPCHK(hipMemGetInfo(&free_b, &total_b));
+      SeededLaunch lr;
+      float ms = 0;
+      if ((rc = seeded_enqueue(b, est, b->rerun, 256, (free_b + held(b->rerun)) / 4 * 3,
+                               ctx->stream, lr)) != MPOA_OK ||
+          (rc = seeded_wait(lr, &ms)) != MPOA_OK)
+        return rc;
       b->kernel_ms += ms;
       HIPCHK(hipMemcpy(res.data(), b->d_res.p, n * sizeof(WinResult), hipMemcpyDeviceToHost));
       again.clear();
@@ -1451,11 +1531,7 @@ int mpoa_batch_execute(mpoa_batch* b) {
   const bool keep_path = b->params.result != 0;
   const bool full = b->params.ab_band != 0 && b->params.band_f > 0;
   std::vector<Estimate> est(n);
-  if (b->params.seeding) {
-    // -S sub-windows always band adaptively: rows as wide as the read
-    for (int w = 0; w < n; w++) est[w] = estimate(b->info[w], b->sc, true);
-    return execute_seeded(b, est);
-  }
+  if (b->params.seeding) return execute_seeded(b, est);
   for (int w = 0; w < n; w++) est[w] = capacity_of(b, w, full);
   if (b->mode == MPOA_MODE_PLAIN) return execute_plain(b, est);
 

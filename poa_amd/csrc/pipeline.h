@@ -302,9 +302,12 @@ int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t s
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream);
 // Device seeding: every read's minimizers (one workgroup per read), before the first commit.
 int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream);
-// Seeded (-S) windows of a batch: the persistent launch, one window per workgroup at a time
-// (256 threads, one workgroup per CU), every read seeded and its sub-windows run in turn.
-int mpoa_launch_seeded(const mpoa::KernelArgs& a, int n_slots, hipStream_t stream);
+// Seeded (-S) windows of a batch: the persistent launch, one window per workgroup at a time,
+// every read seeded and its sub-windows run in turn; nt 64 (a 30 KB tile, several per CU) or
+// 256 (one per CU, room for any window's seeding). 0, -1 on a launch error, -2 for another nt.
+int mpoa_launch_seeded(const mpoa::KernelArgs& a, int n_slots, int nt, hipStream_t stream);
+// Workgroups of the nt-thread seeded tile one CU holds at once (0 if unsupported).
+int mpoa_seeded_occupancy(int nt);
 // Device address of this device's allocation-free scratch array (MPOA_NOALLOC_SCRATCH_BYTES),
 // looked up once per device with that device current. 0, or -1 when the lookup fails.
 int mpoa_scratch_address(int device, void** out);

@@ -2817,10 +2817,17 @@ __device__ void run_window_seeded(const KernelArgs& a, Arena& A, WinShared& S, R
   window_outputs<NT>(a, A, S, w, lds, lds_bytes);
 }
 
-constexpr int SEEDED_LDS = 156 * 1024;
+// LDS of the seeded-batch tiles: the 256-thread tile takes what one CU has (every window
+// fits its seeding); the 64-thread tile a fifth, so that several windows share a CU (a window
+// whose seeding outgrows it re-runs on the 256-thread tile).
+template <int NT>
+struct SeededLds {
+  static constexpr int bytes = NT == 64 ? 30 * 1024 : 156 * 1024;
+};
 
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void k_seeded(KernelArgs a) {
+  constexpr int SEEDED_LDS = SeededLds<NT>::bytes;
   __shared__ WinShared S;
   __shared__ __align__(16) uint8_t lds[SEEDED_LDS];
   using RingT = Ring<RingShape<NT>::RING, RingShape<NT>::WR, RingShape<NT>::QCAP>;
@@ -3161,11 +3168,22 @@ int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mpoa_launch_seeded(const mpoa::KernelArgs& a, int n_slots, hipStream_t stream) {
+int mpoa_launch_seeded(const mpoa::KernelArgs& a, int n_slots, int nt, hipStream_t stream) {
   using namespace mpoa;
   if (n_slots <= 0) return 0;
-  hipLaunchKernelGGL((k_seeded<256, 2>), dim3(n_slots), dim3(256), 0, stream, a);
+  if (nt == 64) hipLaunchKernelGGL((k_seeded<64, 4>), dim3(n_slots), dim3(64), 0, stream, a);
+  else if (nt == 256) hipLaunchKernelGGL((k_seeded<256, 2>), dim3(n_slots), dim3(256), 0, stream, a);
+  else return -2;
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mpoa_seeded_occupancy(int nt) {
+  using namespace mpoa;
+  int blocks = 0;
+  hipError_t e = nt == 64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_seeded<64, 4>, 64, 0)
+               : nt == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_seeded<256, 2>, 256, 0)
+                           : hipErrorInvalidValue;
+  return e == hipSuccess ? blocks : 0;
 }
 
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream) {

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_seed_reads(SeedArgs s) {
                            reinterpret_cast<uint8_t*>(lds8), (int)sizeof(lds8), sh);
 }
 
-// The pieces of the local chains kept in LDS (SEED_ANCHORS ints each).
+// The pieces of the local chains kept in LDS (one int per anchor each).
 struct ChainLds {
   int *tp, *qp, *f, *p, *t, *v;
 };
@@ -311,11 +311,14 @@ __device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev&
   // Local chains (mg_lchain_dp, lchain.cpp:158-238, with dp_chaining's arguments,
   // minimizer.cpp:493-500: max_dist = bw, max_skip 25, max_iter 50, min_cnt 3, min_sc 100).
   ChainLds C;
+  // six arrays of na ints (the picked anchors reuse v: 2 * SEED_SEGS ints), then the sorts
+  const int na = max((n_a + 63) / 64 * 64, 2 * SEED_SEGS);
+  if (6 * na * 4 > lds_bytes) return fail(SEED_WHY_ANCHORS);
   C.tp = reinterpret_cast<int*>(lds);
-  C.qp = C.tp + SEED_ANCHORS; C.f = C.qp + SEED_ANCHORS; C.p = C.f + SEED_ANCHORS;
-  C.t = C.p + SEED_ANCHORS; C.v = C.t + SEED_ANCHORS;
-  uint8_t* slds = reinterpret_cast<uint8_t*>(C.v + SEED_ANCHORS);
-  const int slds_bytes = lds_bytes - 6 * SEED_ANCHORS * 4;
+  C.qp = C.tp + na; C.f = C.qp + na; C.p = C.f + na;
+  C.t = C.p + na; C.v = C.t + na;
+  uint8_t* slds = reinterpret_cast<uint8_t*>(C.v + na);
+  const int slds_bytes = lds_bytes - 6 * na * 4;
   for (int i = tid; i < n_a; i += NT) {
     C.tp[i] = (int)(uint32_t)d.bx[i];
     C.qp[i] = (int)(uint32_t)d.by[i];

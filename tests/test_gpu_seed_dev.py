@@ -160,8 +160,9 @@ def test_seeded_batch_against_oracle(ctx, flags):
     bat.execute()
     got = bat.results()
     assert got.texts == want
-    assert got.stats.reruns == 0, "windows left the seeded launch"
-    assert got.stats.launches == 1
+    # a window past its estimate re-runs in a second seeded launch; few should
+    assert got.stats.reruns <= len(want) // 10, "windows left the seeded launches"
+    assert 1 <= got.stats.launches <= 3
     bat.set_forms(api.FORM_HOST_SEED)
     bat.execute()
     assert bat.results().texts == want
