@@ -193,14 +193,14 @@ int do_topsort(mpoa_graph* g, int op_code) {
 
 // poa() on [rank(beg_id), rank(end_id)]: appends its res entries to `res`.
 int do_poa(mpoa_graph* g, int beg_id, int end_id, const uint8_t* q, int qlen, int ab_band,
-           std::vector<mpoa_res>* res, int* score) {
+           std::vector<mpoa_res>* res, int* score, bool sub_window = false) {
   if (beg_id < 0 || end_id < 0 || beg_id >= g->st.n_nodes || end_id >= g->st.n_nodes) {
     g_err = "node id out of range";
     return MPOA_E_ARG;
   }
-  if (qlen < 0) {
-    // a -S sub-window that ends before it begins in the read (overlapping anchors); minipoa's
-    // vector of negative size aborts there
+  if (qlen < (sub_window ? -1 : 0)) {
+    // a -S sub-window that ends more than one base before it begins in the read (overlapping
+    // anchors), where minipoa's DP reads past its query; length -1 runs (no res entry)
     g_err = "query sub-window of negative length";
     return MPOA_E_EMPTY_WINDOW;
   }
@@ -410,7 +410,8 @@ int mpoa_alignment(mpoa_graph* g, mpoa_seeder* mm, int32_t rid, const char* qseq
     }
     const int sub_len = end_qpos - beg_q;
     int end_id = g->cons[3 * end_tpos];
-    if ((rc = do_poa(g, beg_id, end_id, t.data() + j, sub_len, 1, &r, nullptr)) != MPOA_OK)
+    if ((rc = do_poa(g, beg_id, end_id, t.data() + j, sub_len, 1, &r, nullptr, true)) !=
+        MPOA_OK)
       return rc;
     j += sub_len;
     for (int k = 0; k < q_span; k++, j++) {

@@ -1139,10 +1139,11 @@ int run_seeded(mpoa_batch* b, const char* seq, const int64_t* seq_off,
           const int sink_rank = st.n_nodes - 1;
           int beg_id = 0, beg_rank = 0, beg_q = 0, j = 0, res_pos = 0;
           // <= 0: a sub-window that ends before it begins, in ranks (minipoa writes to an
-          // empty vector) or in the read (anchors overlapping there: minipoa's vector of
-          // negative size aborts with std::length_error)
+          // empty vector) or in the read by more than one base (anchors overlapping there,
+          // where minipoa's DP reads past its query). Length -1 is a DP call that adds no res
+          // entry (the device's null_call_score).
           auto add_seg = [&](int end_id, int end_rank, int sub_len) {
-            if (sub_len < 0) return 0;
+            if (sub_len < -1) return 0;
             SegDesc d{};
             d.beg_id = beg_id;
             d.end_id = end_id;

@@ -2571,6 +2571,35 @@ __device__ __forceinline__ void run_window(const KernelArgs& a, Arena& A, WinSha
 
 #include "seed_dev.h"
 
+// poa() on a -S sub-window of query length -1 (two anchors overlapping by one base in the
+// read, align.cpp:1771-1790): minipoa runs its DP with m = 0, a band of one block of N columns
+// on every row (Ms 0, Me 0), and its backtrack from column 0 takes deletions only, so the
+// call adds no res entry. Its score is H[n-1][0]: row 0 has H 0, and every later row takes the
+// largest of its predecessors' H + o1 and D + e1 (the D source, align.cpp:1484-1506) and of
+// NEG_INF plus the match or mismatch score against the column's N (the M source's block-0
+// diagonal, :1467-1483). One lane; h and d hold n ints each.
+__device__ int null_call_score(const Arena& A, const DpScores& sc, int beg, int n, int* h,
+                               int* d) {
+  h[0] = 0;
+  d[0] = NEG_INF;
+  for (int i = 1; i < n; i++) {
+    const RowRec rr = load_rowrec(A.rowrec, beg + i);
+    int hm = NEG_INF, dv = NEG_INF;
+    for (int k = 0; k < rr.np; k++) {
+      const int prank = k == 0 ? rr.p0 : k == 1 ? rr.p1 : k == 2 ? rr.p2 : A.pred_rank[rr.k0 + k];
+      const int p = prank - beg;
+      if (p < 0) continue;
+      const int b = k < 3 ? (rr.pk >> (3 * k)) & 7 : A.pred_base[rr.k0 + k] & 7;
+      const int pbase = p == 0 ? CODE_N : b;
+      hm = max(hm, NEG_INF + (pbase == CODE_N ? sc.match : sc.mismatch));
+      dv = max(dv, max(d[p] + sc.e1, h[p] + sc.o1));
+    }
+    d[i] = dv;
+    h[i] = max(max(hm, dv), NEG_INF);
+  }
+  return h[n - 1];
+}
+
 __device__ void state_load(WinShared& S, const WinState& st) {
   S.n_nodes = st.n_nodes; S.n_in_e = st.n_in_e; S.n_out_e = st.n_out_e;
   S.status = st.status; S.msa_cols = st.msa_cols; S.cons_len = st.cons_len;
@@ -2639,6 +2668,19 @@ __global__ __launch_bounds__(NT) void k_segments(SeedArgs s) {
     A.rowrec[k] = G.rowrec[k];
   }
   __syncthreads();
+  if (d.qlen == -1) {  // anchors overlapping by one base: no DP, no res entry
+    if (tid == 0) {
+      const int64_t cap = d.dp_len > 0 ? 3 * d.dp_len : 3 * s.seg.caps.dp;
+      out.status = 2LL * S.n_rows <= cap ? MPOA_OK : MPOA_E_OVERFLOW;
+      out.score = out.status == MPOA_OK
+                      ? null_call_score(A, s.graph.scores, S.beg_i, S.n_rows, A.H, A.H + S.n_rows)
+                      : 0;
+      out.rows = S.n_rows; out.qlen = -1; out.width = 0;
+      out.cells = 0; out.edge_cells = 0; out.logical = 0;
+      for (int k = 0; k < PH_N; k++) out.ticks[k] = 0;
+    }
+    return;
+  }
   const WinDesc& wd = s.graph.win[0];
   const int64_t q0 = s.graph.read_off[wd.read0 + s.read] + d.q0;
   load_query<NT>(A, S, R, s.graph.read_seq + q0, d.qlen);
@@ -2791,6 +2833,14 @@ __device__ void run_window_seeded(const KernelArgs& a, Arena& A, WinShared& S, R
           S.n_rows = S.end_i - S.beg_i + 1;
         }
         __syncthreads();
+        if (sd.qlen == -1) {  // anchors overlapping by one base: no DP, no res entry
+          if (tid == 0) {
+            if (2LL * S.n_rows > 3 * a.caps.dp) S.status = MPOA_E_OVERFLOW;
+            else null_call_score(A, a.scores, S.beg_i, S.n_rows, A.H, A.H + S.n_rows);
+          }
+          __syncthreads();
+          continue;
+        }
         load_query<NT>(As, S, R, a.read_seq + q0 + sd.q0, sd.qlen);
         if (S.q_lds) poa_call<NT, CPL, true, false, RingT>(a, As, S, R, adaptive);
         else poa_call<NT, CPL, false, false, RingT>(a, As, S, R, adaptive);
@@ -2887,6 +2937,18 @@ __global__ __launch_bounds__(NT) void k_graph_op(GraphOpArgs g) {
         g.out->status = S.n_rows <= 0 ? MPOA_E_EMPTY_WINDOW : MPOA_E_OVERFLOW;
         g.out->rows = S.n_rows;
         g.out->qlen = g.qlen;
+      }
+      return;
+    }
+    if (g.qlen == -1) {  // a -S sub-window of anchors overlapping by one base: no res entry
+      if (tid == 0) {
+        GraphOpOut& o = *g.out;
+        o.status = 2LL * S.n_rows <= 3 * a.caps.dp ? MPOA_OK : MPOA_E_OVERFLOW;
+        o.score = o.status == MPOA_OK
+                      ? null_call_score(A, a.scores, S.beg_i, S.n_rows, A.H, A.H + S.n_rows)
+                      : 0;
+        o.rows = S.n_rows; o.qlen = -1; o.width = 0;
+        o.cells = 0; o.edge_cells = 0; o.logical = 0;
       }
       return;
     }

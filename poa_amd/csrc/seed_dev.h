@@ -533,8 +533,11 @@ __device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev&
     int beg_id = 0, beg_rank = 0, beg_q = 0, j = 0, res_pos = 0;
     int64_t dp_used = 0;
     if (n_pick >= SEED_SEGS) status = MPOA_E_OVERFLOW, why = SEED_WHY_SEGS;  // n_pick + 1
+    // A sub-window of length -1 (anchors overlapping by one base) is a DP call without res
+    // entries (null_call_score); shorter ones end the window where minipoa reads past its
+    // query (MPOA_E_EMPTY_WINDOW), as does one that ends before it begins in ranks.
     auto add_seg = [&](int end_id, int end_rank, int sub_len) {
-      if (sub_len < 0 || end_rank - beg_rank + 1 <= 0) {
+      if (sub_len < -1 || end_rank - beg_rank + 1 <= 0) {
         status = MPOA_E_EMPTY_WINDOW;
         return;
       }

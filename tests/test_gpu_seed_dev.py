@@ -87,8 +87,9 @@ def test_random_windows_against_oracle(ctx, flags):
         out, calls, how, rc = _outcome(ctx, p, seq, off, 0)
         assert how == 1, f"window {w} left device seeding ({how})"
         if rc == EMPTY and want_rc != 0:
-            # anchors closer than k on the read: a sub-window with a negative query length,
-            # on which minipoa runs poa() past its query's end and exits one way or another
+            # anchors overlapping by more than one base on the read: a sub-window of query
+            # length below -1, where minipoa's DP reads past its query and exits one way or
+            # another; the library stops the window
             pass
         else:
             assert (rc, out) == (want_rc, want), f"window {w}"
@@ -166,4 +167,35 @@ def test_seeded_batch_against_oracle(ctx, flags):
     bat.set_forms(api.FORM_HOST_SEED)
     bat.execute()
     assert bat.results().texts == want
+    bat.free()
+
+
+# Windows of the bench workload (synth.config3(), default seed) whose reads under -S have two
+# anchors overlapping by one base: sub-windows of query length -1, which minipoa runs as DP
+# calls without res entries (window 21 has 27 of them, some over 4 rows).
+OVERLAP = [14, 21, 22, 61, 104, 110, 118, 170, 175, 182, 189, 209, 243, 344, 372, 392]
+
+
+@pytest.mark.gpu
+def test_overlapping_anchors_against_oracle(ctx):
+    full = synth.config3()
+    p = api.Params(seeding=1, result=0)
+    n_null = 0
+    for w in OVERLAP:
+        win = full.window(w)
+        _say("seed_dev overlap window", w)
+        want, ocalls = oracle_lib.window(p, win.seq, win.off)
+        want_calls = [(c.read, c.rows, c.qlen, c.score) for c in ocalls]
+        n_null += sum(1 for c in want_calls if c[2] == -1)
+        for forms in (0, api.FORM_HOST_SEED):
+            out, calls, how = _run(ctx, p, win.seq, win.off, forms)
+            assert how == (1 if forms == 0 else 0)
+            assert out == want, f"window {w} forms {forms}"
+            assert calls == want_calls, f"window {w} forms {forms}"
+    assert n_null >= 50
+    sub = full.subset(OVERLAP)
+    bat = api.Batch(ctx, p, sub.win_off, sub.seq, sub.seq_off)
+    bat.execute()
+    assert bat.results().texts == [oracle_lib.window(p, full.window(w).seq,
+                                                     full.window(w).off)[0] for w in OVERLAP]
     bat.free()
