@@ -970,27 +970,40 @@ int run_seeded_dev(mpoa_batch* b, const int64_t* seq_off, std::vector<mpoa_call_
   sa.state = d_state.p;
   sa.use_dev = 1;
   sa.dev = d;
-  if (mpoa_launch_seed_reads(sa, ctx->stream) != 0) {
-    g_err = "read sketch launch failed";
-    return MPOA_E_DEVICE;
-  }
+  // The whole launch sequence (the copies and clears above stay outside) is captured as one
+  // hipGraph and launched once, unless graph replay is off for the batch.
+  const bool capture = b->use_graph;
+  if (capture) HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const char* lerr = nullptr;
+  if (mpoa_launch_seed_reads(sa, ctx->stream) != 0) lerr = "read sketch launch failed";
   int path0 = 0;
-  for (int r = 0; r < n; r++) {
+  for (int r = 0; r < n && !lerr; r++) {
     sa.read = r;
     sa.path0 = path0;
     sa.segs = d.segs + (size_t)r * mpoa::SEED_SEGS;
     sa.seg_res = d_segres.p + (size_t)r * mpoa::SEED_SEGS;
     sa.splice = d.splice + (size_t)r * 2 * mpoa::SEED_SPLICE;
-    if (r > 0 && mpoa_launch_segments(sa, 128, 4, ctx->stream) != 0) {
-      g_err = "sub-window launch failed";
-      return MPOA_E_DEVICE;
-    }
-    if (mpoa_launch_commit(sa, ctx->stream) != 0) {
-      g_err = "commit launch failed";
-      return MPOA_E_DEVICE;
-    }
+    if (r > 0 && mpoa_launch_segments(sa, 128, 4, ctx->stream) != 0)
+      lerr = "sub-window launch failed";
+    else if (mpoa_launch_commit(sa, ctx->stream) != 0)
+      lerr = "commit launch failed";
     b->launches += r > 0 ? 2 : 1;
     path0 += (int)(seq_off[r + 1] - seq_off[r]);
+  }
+  if (capture) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (!lerr && e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (!lerr && e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+    if (!lerr && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!lerr && e != hipSuccess) lerr = hipGetErrorString(e);
+  }
+  if (lerr) {
+    g_err = lerr;
+    return MPOA_E_DEVICE;
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   std::vector<int> iv(i_words);
