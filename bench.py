@@ -172,10 +172,12 @@ def single_window(ctx, cores: int, with_cpu: bool, log) -> dict:
     return out
 
 
-def seeded_batch(ctx, batch, with_cpu: bool, cpu_sample: int, log) -> dict:
+def seeded_batch(ctx, batch, with_cpu: bool, cpu_sample: int, cores: int, log) -> dict:
     """The same windows under -S through the batch API (k_seeded: one workgroup per window
-    seeds every read itself and runs its sub-windows in turn), best of two executes; the
-    oracle on the first cpu_sample windows on one host core, outputs compared."""
+    seeds every read itself and runs its sub-windows in turn), best of two executes; upstream
+    minipoa -S on 1,000 evenly spaced windows, one process per window (-t 1), `cores` at a time
+    (windows it fails on count, with their exit status); the oracle on the first cpu_sample
+    windows on one host core; outputs and failures compared."""
     import collections
     from poa_amd import api
     params = api.Params(seeding=1, result=0)
@@ -216,9 +218,39 @@ def seeded_batch(ctx, batch, with_cpu: bool, cpu_sample: int, log) -> dict:
             else:
                 diverged += 1
         cw = time.perf_counter() - t0
-        rec["cpu_baseline"] = {"value": round(m / cw, 2), "unit": "windows/s", "cores": 1,
-                               "kind": "port", "sample": f"first {m} windows, -S -r 0",
-                               "windows_identical": same, "windows_differing": diverged}
+        rec["oracle_sample"] = {"value": round(m / cw, 2), "unit": "windows/s", "cores": 1,
+                                "kind": "port", "sample": f"first {m} windows, -S -r 0",
+                                "windows_identical": same, "windows_differing": diverged}
+    if with_cpu and os.path.exists(REF_BIN):
+        idx = np.unique(np.linspace(0, n - 1, num=min(1000, n)).round().astype(int))
+        tmp = tempfile.mkdtemp(prefix="mpoa_cpu_s_")
+        try:
+            paths = []
+            for w in idx:
+                pth = os.path.join(tmp, f"w{w}.fa")
+                batch.window(int(w)).to_fasta(pth)
+                paths.append(pth)
+
+            def run(pth):
+                return subprocess.run([REF_BIN, pth, "-S", "-t", "1"], capture_output=True)
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as ex:
+                outs = list(ex.map(run, paths))
+            wall = time.perf_counter() - t0
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        ok = [o.returncode == 0 for o in outs]
+        same = sum(1 for i, w in enumerate(idx)
+                   if ok[i] and status[int(w)] == 0 and outs[i].stdout.decode() == texts[int(w)])
+        rec["cpu_baseline"] = {
+            "value": round(len(idx) / wall, 2), "unit": "windows/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"{len(idx)} of the {n} windows (evenly spaced), one minipoa -S -t 1 "
+                      f"process per window, {cores} at a time; wall {wall:.2f} s",
+            "upstream_finished": int(sum(ok)),
+            "finished_identical_to_gpu": int(same)}
+        rec["gpu_over_cpu"] = round(rec["windows_per_s"] / rec["cpu_baseline"]["value"], 2)
     log(f"seeded batch: {rec['windows_per_s']} windows/s, statuses {rec['statuses']}")
     return rec
 
@@ -569,7 +601,7 @@ def main() -> int:
     if not args.no_single:
         line["single_window"] = single_window(ctx, cores, not args.no_cpu, log)
     if not args.no_seeded:
-        line["seeded_batch"] = seeded_batch(ctx, batch, not args.no_cpu, 200, log)
+        line["seeded_batch"] = seeded_batch(ctx, batch, not args.no_cpu, 200, cores, log)
     ctx.close()
     print(json.dumps(line), flush=True)
     return 0
