@@ -394,6 +394,15 @@ int mpoa_seed_anchors(const mpoa_params* params, int32_t n_reads, const char* se
                       const int64_t* seq_off, int32_t rid, const char* cons, int32_t cons_len,
                       uint64_t* out, int64_t cap, int64_t* n);
 
+/* The same anchors computed on the device of ctx (the device seeding: minimizers, sorts,
+ * anchors, local and global chains as the -S windows run them, on one workgroup), in the same
+ * format. MPOA_E_OVERFLOW past the device form's capacities (more than 4,096 anchors, 8,192
+ * minimizers in one sort), where mpoa_seed_anchors still answers. */
+int mpoa_seed_anchors_device(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads,
+                             const char* seq, const int64_t* seq_off, int32_t rid,
+                             const char* cons, int32_t cons_len, uint64_t* out, int64_t cap,
+                             int64_t* n);
+
 /* ------------------------------------------------------------------------------------
  * Learned scheduling. minipoa sizes its DP buffer by growing it on demand per thread
  * (src/mem_alloc_utils.cpp:11-40, used at src/align.cpp:1395-1409) and has no scheduler.

@@ -38,7 +38,7 @@ EXPORTS = [
     "mpoa_graph_add_path", "mpoa_graph_topsort", "mpoa_graph_output", "mpoa_graph_export",
     "mpoa_graph_read_gfa", "mpoa_reads_load", "mpoa_reads_parse", "mpoa_reads_free",
     "mpoa_reads_view", "mpoa_batch_set_predictions",
-    "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
+    "mpoa_batch_set_schedule", "mpoa_seed_anchors", "mpoa_seed_anchors_device", "mpoa_sched_features", "mpoa_mlp_create", "mpoa_mlp_destroy", "mpoa_mlp_n_params",
     "mpoa_mlp_get_weights", "mpoa_mlp_set_weights", "mpoa_mlp_forward", "mpoa_mlp_gradients",
     "mpoa_mlp_train", "mpoa_mlp_error",
 ]

@@ -1867,6 +1867,109 @@ int mpoa_seed_anchors(const mpoa_params* params, int32_t n_reads, const char* se
   return MPOA_OK;
 }
 
+int mpoa_seed_anchors_device(mpoa_ctx* ctx, const mpoa_params* params, int32_t n_reads,
+                             const char* seq, const int64_t* seq_off, int32_t rid,
+                             const char* cons, int32_t cons_len, uint64_t* out, int64_t cap,
+                             int64_t* n) {
+  if (!ctx || !params || !seq || !seq_off || rid < 0 || rid >= n_reads || cons_len < 0 ||
+      (cap > 0 && !out) || params->k < 1 || params->k > 28 || params->mm_w < 1 ||
+      params->mm_w > 255)
+    return MPOA_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  // the reads go up as one window; the consensus as codes
+  mpoa_params pp = *params;
+  pp.seeding = 0;
+  pp.progressive = 0;
+  int32_t win_off[2] = {0, n_reads};
+  mpoa_batch* b = nullptr;
+  int rc = mpoa_batch_prepare(ctx, &pp, 1, win_off, seq, seq_off, nullptr, &b);
+  if (rc != MPOA_OK) return rc;
+  const int w = params->mm_w, k = params->k;
+  std::vector<int64_t> rm_off(n_reads + 1, 0);
+  for (int r = 0; r < n_reads; r++)
+    rm_off[r + 1] = rm_off[r] + (seq_off[r + 1] - seq_off[r]) + 2 * w + 8;
+  const int64_t qtot = seq_off[n_reads] - seq_off[0];
+  const int cons_cap = std::max(cons_len, 1);
+  const size_t rm = (size_t)rm_off[n_reads];
+  DevBuf<uint64_t> u_buf;
+  DevBuf<int> i_buf;
+  DevBuf<int64_t> d_rm_off;
+  DevBuf<uint8_t> d_cons;
+  const size_t u_words = 4 * rm + (size_t)qtot + 5 * (size_t)cons_cap +
+                         10 * (size_t)mpoa::SEED_ANCHORS;
+  const size_t i_words = (size_t)n_reads + 2;
+  if ((rc = u_buf.alloc(u_words)) != MPOA_OK || (rc = i_buf.alloc(i_words)) != MPOA_OK ||
+      (rc = d_rm_off.alloc(n_reads + 1)) != MPOA_OK || (rc = d_cons.alloc(cons_cap)) != MPOA_OK) {
+    mpoa_batch_free(b);
+    return rc;
+  }
+  std::vector<uint8_t> cc((size_t)cons_cap, 4);
+  for (int j = 0; j < cons_len; j++) cc[j] = code_of_char((unsigned char)cons[j]);
+  HIPCHK(hipMemcpy(d_cons.p, cc.data(), cc.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_rm_off.p, rm_off.data(), rm_off.size() * sizeof(int64_t),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(i_buf.p, 0, i_words * sizeof(int)));
+  mpoa::SeedDev d{};
+  uint64_t* u = u_buf.p;
+  auto take_u = [&](size_t m) { uint64_t* p = u; u += m; return p; };
+  d.rmx = take_u(rm); d.rmy = take_u(rm); d.rtx = take_u(rm); d.rty = take_u(rm);
+  d.hx = take_u((size_t)qtot + cons_cap);
+  d.cmx = take_u(cons_cap); d.cmy = take_u(cons_cap);
+  d.ctx = take_u(cons_cap); d.cty = take_u(cons_cap);
+  d.ax = take_u(mpoa::SEED_ANCHORS); d.ay = take_u(mpoa::SEED_ANCHORS);
+  d.bx = take_u(mpoa::SEED_ANCHORS); d.by = take_u(mpoa::SEED_ANCHORS);
+  d.zx = take_u(mpoa::SEED_ANCHORS); d.zy = take_u(mpoa::SEED_ANCHORS);
+  d.zsx = take_u(mpoa::SEED_ANCHORS); d.zsy = take_u(mpoa::SEED_ANCHORS);
+  d.pick_out = take_u(2 * (size_t)mpoa::SEED_ANCHORS);
+  d.rm_cnt = i_buf.p;
+  d.n_pick_out = i_buf.p + n_reads;
+  d.flag = i_buf.p + n_reads + 1;
+  d.rm_off = d_rm_off.p;
+  d.hx_cons = qtot;
+  d.cons_cap = cons_cap;
+  d.cons_codes = d_cons.p;
+  d.dp_cap = INT64_MAX;
+  d.k = k;
+  d.w = w;
+  d.bw = 1000;  // main.cpp:65
+  d.poa_w = params->poa_w;
+  d.pen_gap = (float)(0.01 * k);  // minimizer.cpp:498
+  d.pen_skip = (float)(0.0 * 0.01 * k);
+  SeedArgs sa{};
+  sa.graph.read_seq = b->d_seq.p;
+  sa.graph.read_off = b->d_off.p;
+  sa.graph.win = b->d_win.p;
+  sa.n_reads = n_reads;
+  sa.read = rid;
+  sa.cons_cap = cons_len;
+  sa.dev = d;
+  if (mpoa_launch_seed_reads(sa, ctx->stream) != 0 ||
+      mpoa_launch_seed_anchors(sa, ctx->stream) != 0) {
+    mpoa_batch_free(b);
+    g_err = "seeding launch failed";
+    return MPOA_E_DEVICE;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int got[2];
+  HIPCHK(hipMemcpy(got, i_buf.p + n_reads, sizeof(got), hipMemcpyDeviceToHost));
+  mpoa_batch_free(b);
+  if (got[0] < 0 || got[1] != 0 || got[0] > mpoa::SEED_ANCHORS) {
+    g_err = "past the device seeding's capacities (mpoa_seed_anchors seeds on the host)";
+    return MPOA_E_OVERFLOW;
+  }
+  std::vector<uint64_t> picks(2 * (size_t)std::max(got[0], 1));
+  if (got[0] > 0)
+    HIPCHK(hipMemcpy(picks.data(), d.pick_out, 2 * (size_t)got[0] * sizeof(uint64_t),
+                     hipMemcpyDeviceToHost));
+  if (n) *n = got[0];
+  // picked last anchor first: the sub-windows take them in read order
+  for (int i = 0; i < got[0] && i < cap; i++) {
+    out[2 * i] = picks[2 * (size_t)(got[0] - 1 - i)];
+    out[2 * i + 1] = picks[2 * (size_t)(got[0] - 1 - i) + 1];
+  }
+  return MPOA_OK;
+}
+
 int mpoa_batch_launch_stats(mpoa_batch* b, mpoa_launch_stat* out, int32_t cap, int32_t* n) {
   if (!b || (cap > 0 && !out)) return MPOA_E_ARG;
   const int k = (int)b->lstats.size();

@@ -227,6 +227,11 @@ struct SeedDev {
   int64_t dp_cap;           // int32 words of dp_pool for one read's sub-windows (H, D, I each)
   int k, w, bw, poa_w;
   float pen_gap, pen_skip;
+  // anchors only (k_seed_anchors): the consensus codes, and the picked anchors out (x, y
+  // pairs as collect_anchors_bycons returns them, last anchor first; SEED_ANCHORS at most)
+  const uint8_t* cons_codes;
+  uint64_t* pick_out;
+  int* n_pick_out;
 };
 
 struct SeedArgs {
@@ -302,6 +307,9 @@ int mpoa_launch_segments(const mpoa::SeedArgs& s, int nt, int cpl, hipStream_t s
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream);
 // Device seeding: every read's minimizers (one workgroup per read), before the first commit.
 int mpoa_launch_seed_reads(const mpoa::SeedArgs& s, hipStream_t stream);
+// The anchors of read s.read on the consensus s.dev.cons_codes (s.cons_cap bases), after
+// mpoa_launch_seed_reads: one workgroup, into s.dev.pick_out / n_pick_out.
+int mpoa_launch_seed_anchors(const mpoa::SeedArgs& s, hipStream_t stream);
 // Seeded (-S) windows of a batch: the persistent launch, one window per workgroup at a time,
 // every read seeded and its sub-windows run in turn; nt 64 (a 30 KB tile, several per CU) or
 // 256 (one per CU, room for any window's seeding). 0, -1 on a launch error, -2 for another nt.

@@ -2867,6 +2867,25 @@ __device__ void run_window_seeded(const KernelArgs& a, Arena& A, WinShared& S, R
   window_outputs<NT>(a, A, S, w, lds, lds_bytes);
 }
 
+// The anchors of read s.read on a given consensus (mpoa_seed_anchors_device): the seeding of
+// seed_next on a graph-less consensus, stopping at the picked anchors.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_seed_anchors(SeedArgs s) {
+  __shared__ WinShared S;
+  __shared__ int4 lds4[(156 * 1024) / 16];
+  Arena A{};
+  A.cons_ch = const_cast<uint8_t*>(s.dev.cons_codes);
+  if (threadIdx.x == 0) {
+    S.status = MPOA_OK;
+    S.n_nodes = 3;  // any graph with bases
+    S.cons_len = s.cons_cap;
+  }
+  __syncthreads();
+  seed_next<NT>(s.graph, s.graph.win[0], s.dev, s.read, 0, A, S,
+                reinterpret_cast<uint8_t*>(lds4), (int)sizeof(lds4));
+  if (threadIdx.x == 0 && S.status != MPOA_OK) *s.dev.n_pick_out = -1;
+}
+
 // LDS of the seeded-batch tiles: the 256-thread tile takes what one CU has (every window
 // fits its seeding); the 64-thread tile a fifth, so that several windows share a CU (a window
 // whose seeding outgrows it re-runs on the 256-thread tile).
@@ -3246,6 +3265,12 @@ int mpoa_seeded_occupancy(int nt) {
                : nt == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_seeded<256, 2>, 256, 0)
                            : hipErrorInvalidValue;
   return e == hipSuccess ? blocks : 0;
+}
+
+int mpoa_launch_seed_anchors(const mpoa::SeedArgs& s, hipStream_t stream) {
+  using namespace mpoa;
+  hipLaunchKernelGGL((k_seed_anchors<256>), dim3(1), dim3(256), 0, stream, s);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int mpoa_launch_commit(const mpoa::SeedArgs& s, hipStream_t stream) {

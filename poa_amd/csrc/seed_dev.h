@@ -257,8 +257,10 @@ __device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev&
     if (tid == 0) {
       atomicOr(d.flag, why);
       S.status = MPOA_E_OVERFLOW;
-      d.n_seg[ri] = 0;
-      d.n_splice[ri] = 0;
+      if (d.n_seg) {  // absent when only the anchors are asked for
+        d.n_seg[ri] = 0;
+        d.n_splice[ri] = 0;
+      }
     }
     __syncthreads();
     return false;
@@ -454,9 +456,17 @@ __device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev&
   SegDesc* segs = d.segs + (int64_t)ri * SEED_SEGS;
   int* splice = d.splice + (int64_t)ri * 2 * SEED_SPLICE;
   if (tid == 0) {
-    int n_pick = 0;
-    auto take = [&](int ct, int cq) {
-      if (n_pick < SEED_SEGS) {
+    int n_pick = 0, filter_num = 0;
+    // a picked anchor; `mark`: fewer than min_w / 10 anchors were passed over since the one
+    // picked before it (chain_dp's band-mode flag, bit 40 of y; the sub-windows ignore it)
+    auto take = [&](int ct, int cq, bool mark) {
+      if (d.pick_out) {
+        if (n_pick < SEED_ANCHORS) {
+          d.pick_out[2 * n_pick] = (uint64_t)wd.n_reads << 32 | (uint32_t)ct;
+          d.pick_out[2 * n_pick + 1] =
+              (uint64_t)k << 32 | (uint32_t)cq | (mark ? 1ull << 40 : 0ull);
+        }
+      } else if (n_pick < SEED_SEGS) {
         pick[2 * n_pick] = ct;
         pick[2 * n_pick + 1] = cq;
       }
@@ -508,8 +518,11 @@ __device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev&
           const int ct = ctp[i], cq = cqp[i];
           if (!(ct > pet && cq > peq)) break;
           if (last_t - ct >= min_w && last_q - cq >= min_w) {
-            take(ct, cq);
+            take(ct, cq, filter_num < min_w / 10);
+            filter_num = 0;
             last_t = ct, last_q = cq;
+          } else {
+            filter_num++;
           }
           i--;
           if (i == (int)(cy >> 32)) break;
@@ -520,13 +533,28 @@ __device__ bool seed_next(const KernelArgs& a, const WinDesc& wd, const SeedDev&
       while (i >= 0) {
         const int ct = ctp[i], cq = cqp[i];
         if (last_t - ct >= min_w && last_q - cq >= min_w) {
-          take(ct, cq);
+          take(ct, cq, filter_num < min_w / 10);
+          filter_num = 0;
           last_t = ct, last_q = cq;
+        } else {
+          filter_num++;
         }
         if (i == (int)(cy >> 32)) break;
         i--;
       }
     }
+    ctl[3] = n_pick;
+    if (d.pick_out) {  // anchors only
+      *d.n_pick_out = n_pick;
+      ctl[2] = 1;
+    } else {
+      ctl[2] = 0;
+    }
+  }
+  __syncthreads();
+  if (ctl[2]) return true;
+  if (tid == 0) {
+    const int n_pick = min(ctl[3], SEED_SEGS + 1);
     // Sub-windows: up to each picked anchor's first base, then its k-1 bases spliced in as
     // the consensus nodes they matched, the last one opening the next sub-window.
     int status = MPOA_OK, why = 0, n_seg = 0, n_sp = 0;
