@@ -147,9 +147,10 @@ def _finishing_windows(p, seed: int, n: int):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("flags", [dict(seeding=1, result=0), dict(seeding=1, result=1),
+                                   dict(seeding=1, result=2),
                                    dict(seeding=1, k=15, mm_w=5, result=3),
                                    dict(seeding=1, progressive=1, result=1)],
-                         ids=["S_r0", "S_r1", "S_k15w5_r3", "S_p_r1"])
+                         ids=["S_r0", "S_r1", "S_r2", "S_k15w5_r3", "S_p_r1"])
 def test_seeded_batch_against_oracle(ctx, flags):
     """-S windows in the batch API: one workgroup per window seeds every read itself and runs
     its sub-windows in turn (k_seeded); the host-seeded form gives the same texts."""
