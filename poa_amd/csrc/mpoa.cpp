@@ -429,12 +429,19 @@ void assign_bins(mpoa_batch* b) {
   b->wide.cpl = 2;
   // One window alone whose band passes the 512 threads' 1,024 columns: four columns a thread,
   // so that every row is one pass (two passes cost twice the row's scan and barriers).
-  if (!b->split)
+  // A band of at most 448 columns with a query the 256-thread tile holds in LDS: that tile,
+  // whose rows need half the waves at each barrier and scan (a 2 kb, 100-read window: DP
+  // 1.13 s against 1.35 s on the 512-thread tile).
+  if (!b->split) {
+    bool narrow = !b->wide.wins.empty();
     for (int w : b->wide.wins) {
       const WinInfo& wi = b->info[w];
       const int width = b->pred_width.empty() ? width_of(wi) : (int)b->pred_width[w];
       if (width > 1024) b->wide.cpl = 4;
+      narrow = narrow && width <= 448 && wi.qmax <= 4000;
     }
+    if (narrow) b->wide.nt = 256;
+  }
   b->bin_of.assign(b->n_windows, 0);
   // The costliest small-tile windows join the medium launch up to move_top windows (default
   // half a window per CU): they set the makespan, and the larger tile shortens their rows.
