@@ -295,7 +295,10 @@ int mpoa_batch_set_tile(mpoa_batch* b, int32_t threads, int32_t cols);
 int mpoa_batch_set_schedule(mpoa_batch* b, int32_t small_qmax, int32_t small_width,
                             int32_t move_top, int32_t small_nodes);
 /* Runs the whole per-window pipeline on the device for every window; inputs are already
- * resident. Synchronous. */
+ * resident. Synchronous. Returns MPOA_OK when every window finished, else the status of the
+ * first window that did not (minipoa's own failures: MPOA_E_BAND, MPOA_E_BACKTRACK,
+ * MPOA_E_EMPTY_WINDOW); every window's status is in mpoa_batch_window_stats and the finished
+ * windows' texts are in mpoa_batch_results either way. */
 int mpoa_batch_execute(mpoa_batch* b);
 /* Copies the per-window output text (minipoa's stdout for that window) to out; window w is
  * out[out_off[w] .. out_off[w+1]). */
