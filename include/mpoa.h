@@ -344,12 +344,16 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode);
  * eight predecessors in its block); MPOA_FORM_WIDE, int32 storage of the DP matrices for every
  * call (a call whose values do not fit int16: minipoa's own width); MPOA_FORM_HOST_SEED, -S
  * reads seeded on the host between launches (a window that outgrows the device seeding's
- * capacities, or ties among more than 64 equal minimizer or anchor keys). 0 (the default) restores
+ * capacities, or ties among more than 64 equal minimizer or anchor keys); MPOA_FORM_BT_RANKS, the
+ * wave backtrack's runs of match moves along consecutive ranks instead of along each row's
+ * heaviest predecessor (a call whose query and graph do not both fit the LDS left after the
+ * DP). 0 (the default) restores
  * the normal choice. Results do not change; only speed does. mpoa_set_forms sets the forms of
  * the batches the context creates afterwards, mpoa_align_reads' included. */
 enum {
   MPOA_FORM_SORT_HBM = 1, MPOA_FORM_SORT_LANE = 2, MPOA_FORM_BT_LANE = 4, MPOA_FORM_ADD_LANE = 8,
-  MPOA_FORM_CONS_ROUNDS = 16, MPOA_FORM_WIDE = 32, MPOA_FORM_HOST_SEED = 64
+  MPOA_FORM_CONS_ROUNDS = 16, MPOA_FORM_WIDE = 32, MPOA_FORM_HOST_SEED = 64,
+  MPOA_FORM_BT_RANKS = 128
 };
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms);
 int mpoa_set_forms(mpoa_ctx* ctx, int32_t forms);

@@ -22,6 +22,7 @@ MPOA_E_OUT_CAP = -4
 FORM_SORT_HBM, FORM_SORT_LANE, FORM_BT_LANE, FORM_ADD_LANE, FORM_CONS_ROUNDS = 1, 2, 4, 8, 16
 FORM_WIDE = 32  # int32 storage of the DP matrices for every call (no int16 form)
 FORM_HOST_SEED = 64  # -S reads seeded on the host between launches (no device seeding)
+FORM_BT_RANKS = 128  # backtrack runs along consecutive ranks (not the heaviest-predecessor chain)
 ERRORS = {
     -1: "E_ARG", -2: "E_BAND", -3: "E_BACKTRACK", -4: "E_OUT_CAP", -5: "E_EMPTY_WINDOW",
     -6: "E_OOM", -7: "E_DEVICE", -8: "E_OVERFLOW", -9: "E_SCRATCH",

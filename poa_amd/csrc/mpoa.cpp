@@ -1770,7 +1770,7 @@ int mpoa_batch_set_mode(mpoa_batch* b, int32_t mode) {
 
 constexpr int kAllForms = MPOA_FORM_SORT_HBM | MPOA_FORM_SORT_LANE | MPOA_FORM_BT_LANE |
                           MPOA_FORM_ADD_LANE | MPOA_FORM_CONS_ROUNDS | MPOA_FORM_WIDE |
-                          MPOA_FORM_HOST_SEED;
+                          MPOA_FORM_HOST_SEED | MPOA_FORM_BT_RANKS;
 
 int mpoa_batch_set_forms(mpoa_batch* b, int32_t forms) {
   if (!b || (forms & ~kAllForms) != 0) return MPOA_E_ARG;

@@ -1473,9 +1473,110 @@ __device__ __forceinline__ int bt_run(const Arena& A, const WinShared& S, const 
   return L;
 }
 
+// The heaviest in-window predecessor of local row r (the first of the largest in-weight, the
+// one the match-only move takes when it is a candidate, align.cpp:1591-1626), packed as its
+// local rank | base << 24 | (in-weight >= ind/10) << 27; -1 for row 0, a row without one, or a
+// row with more than 63 predecessors (the one-lane backtrack's).
+__device__ __forceinline__ int heaviest_pred(const Arena& A, int beg, int r) {
+  if (r == 0) return -1;
+  const RowRec rec = load_rowrec(A.rowrec, beg + r);
+  if (rec.np < 1 || rec.np > 63) return -1;
+  int best = -1, bw = -1, b8 = 0;
+  for (int k = 0; k < rec.np; k++) {
+    int prank, w, b;
+    if (k < 3) {
+      prank = k == 0 ? rec.p0 : k == 1 ? rec.p1 : rec.p2;
+      w = k == 0 ? rec.w0 : k == 1 ? rec.w1 : rec.w2;
+      b = ((rec.pk >> (3 * k)) & 7) | (((rec.pk >> (9 + k)) & 1) << 7);
+    } else {
+      prank = A.pred_rank[rec.k0 + k];
+      w = A.pred_w[rec.k0 + k];
+      b = A.pred_base[rec.k0 + k];
+    }
+    if (prank < beg || w <= bw) continue;
+    bw = w;
+    best = prank - beg;
+    b8 = b;
+  }
+  return best < 0 ? -1 : best | ((b8 & 7) << 24) | (((b8 >> 7) & 1) << 27);
+}
+
+// A run of match moves along the heaviest-predecessor chain, checked by the whole wave at
+// once: lane t takes row r_t (r_0 = i, r_{t+1} = the heaviest in-window predecessor of r_t,
+// from hp) at column acj - t and checks that minipoa's first rule takes the move to
+// (r_{t+1}, acj - t - 1): r_{t+1} is a candidate (its base is the query's, the column is
+// inside both bands, H equals its diagonal H plus the match score), so as the heaviest it is
+// the one chosen, and its in-weight passes the ind/10 rule. Otherwise as bt_run.
+template <class T>
+__device__ __forceinline__ int bt_chain(const Arena& A, const WinShared& S, const T* H,
+                                        const int* hp, int match, int i, int acj, int op, int K,
+                                        int* log, int* ni, int* nbs, int* nbe, int* noff,
+                                        RowRec* nrr) {
+  const int lane = threadIdx.x & 63;
+  const int beg = S.beg_i;
+  int r = -1, hk = -1;
+  for (int t = 0, c = i; t <= K; t++) {  // the chain, one LDS read a step
+    const int v = c >= 0 ? hp[c] : -1;
+    if (lane == t) {
+      r = c;
+      hk = v;
+    }
+    c = v < 0 ? -1 : (v & 0xffffff);
+  }
+  const int a = acj - lane;
+  const bool live = lane <= K && r >= 0 && a >= 0;
+  int bs = 0, be = 0, off = 0, qc = -1;
+  RowRec rec{};
+  if (live) {
+    bs = A.Bs[r];
+    be = A.Be[r];
+    off = (int)A.row_off[r];
+    rec = load_rowrec(A.rowrec, beg + r);
+    if (a > 0) qc = A.qseq[a - 1];
+  }
+  const bool in_band = live && a >= bs * Q && a <= (be - 1) * Q;
+  T rh = 0;  // widened after the branch: no wait inside it
+  if (in_band) rh = H[off + a - bs * Q];
+  const int h = widen(rh);
+  const int up_bs = __shfl_down(bs, 1, 64), up_be = __shfl_down(be, 1, 64);
+  const int up_h = __shfl_down(h, 1, 64);
+  const int p = hk & 0xffffff;
+  bool ok = in_band && r >= 1 && a >= 1 && lane < K && hk >= 0;
+  if (ok) {
+    const int pbase = p == 0 ? CODE_N : (hk >> 24) & 7;
+    const int pj = a - up_bs * Q;  // lane t + 1 holds row p: its band and diagonal cell
+    ok = pbase == qc && ((hk >> 27) & 1) && pj - 1 >= 0 &&
+         pj - 1 < (up_be - up_bs - 1) * Q && h == up_h + match;
+  }
+  if (lane == 0 && !(op & M_OP)) ok = false;
+  const uint64_t good = __ballot(ok);
+  const int L = (int)__builtin_ctzll(~good);
+  if (lane < L) log[a - 1] = p;
+  *ni = __builtin_amdgcn_readlane(r, L);
+  *nbs = __builtin_amdgcn_readlane(bs, L);
+  *nbe = __builtin_amdgcn_readlane(be, L);
+  *noff = __builtin_amdgcn_readlane(off, L);
+  nrr->row_off = __builtin_amdgcn_readlane(rec.row_off, L);
+  nrr->bsbe = __builtin_amdgcn_readlane(rec.bsbe, L);
+  nrr->k0 = __builtin_amdgcn_readlane(rec.k0, L);
+  nrr->np = __builtin_amdgcn_readlane(rec.np, L);
+  nrr->p0 = __builtin_amdgcn_readlane(rec.p0, L);
+  nrr->p1 = __builtin_amdgcn_readlane(rec.p1, L);
+  nrr->p2 = __builtin_amdgcn_readlane(rec.p2, L);
+  nrr->pk = __builtin_amdgcn_readlane(rec.pk, L);
+  nrr->w0 = __builtin_amdgcn_readlane(rec.w0, L);
+  nrr->w1 = __builtin_amdgcn_readlane(rec.w1, L);
+  nrr->w2 = __builtin_amdgcn_readlane(rec.w2, L);
+  nrr->last = __builtin_amdgcn_readlane(rec.last, L);
+  return L;
+}
+
+// hp: the heaviest in-window predecessor of every row (heaviest_pred) in LDS, or null; with
+// it, runs follow the heaviest-predecessor chain (bt_chain) and are tried after every match
+// move, else they follow consecutive ranks (bt_run).
 template <class T>
 __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc,
-                                              const DpMats<T>& X, int* log) {
+                                              const DpMats<T>& X, int* log, const int* hp) {
   const int lane = threadIdx.x & 63;
   const int n = S.n_rows, m = S.m, beg = S.beg_i;
   const int match = sc.match, mismatch = sc.mismatch, o1 = sc.o1, e1 = sc.e1;
@@ -1487,22 +1588,28 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
   int p = -1, pw = 0, pbase = 0, prule = 0, pbs = 0, pbe = 0, poff = 0;
   RowRec prr{};
   bool setup = true;
-  // Runs of match moves along consecutive ranks (bt_run) are tried after two match moves in a
-  // row to the previous rank (where a run is likely to go on), and again after a run that used
-  // its whole width; the width starts at 8 moves and doubles after such a run (up to 63), since
-  // a run's loads cost more the more lanes take part.
+  // Runs of match moves are tried along the heaviest-predecessor chain after every match move
+  // when hp is given (about 88% of the moves of these graphs lie in such runs, 9 moves long on
+  // average), else along consecutive ranks after two match moves in a row to the previous rank
+  // (where a run is likely to go on); and again after a run that used its whole width. The
+  // width starts at 8 moves and doubles after such a run (up to 63), since a run's loads cost
+  // more the more lanes take part.
   bool run = false;
   int width = 8, streak = 0;
   for (;;) {
     if (run && (op & M_OP) && acj > 0 && i > 0) {
-      int nbs, nbe, noff;
+      int ni = i, nbs, nbe, noff;
       RowRec nrr;
-      const int L = bt_run(A, S, X.H, match, i, acj, op, width, log, &nbs, &nbe, &noff, &nrr);
+      const int L = hp ? bt_chain(A, S, X.H, hp, match, i, acj, op, width, log, &ni, &nbs,
+                                  &nbe, &noff, &nrr)
+                       : bt_run(A, S, X.H, match, i, acj, op, width, log, &nbs, &nbe, &noff,
+                                &nrr);
+      if (!hp) ni = i - L;  // consecutive ranks
       run = L == width;  // a full run may go on; otherwise the next move is not such a match
       width = run ? min(2 * width, 63) : 8;
       if (!run) streak = 0;
       if (L > 0) {
-        i -= L;
+        i = ni;
         acj -= L;
         op = ALL_OP;
         bs_i = nbs;
@@ -1615,8 +1722,12 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
       }
     }
     if (kind == 0) return MPOA_E_BACKTRACK;
-    streak = matched && __builtin_amdgcn_readlane(p, k) == i - 1 ? streak + 1 : 0;
-    run = streak >= 2;
+    if (hp) {
+      run = matched;  // most match moves of these graphs start or continue a chain run
+    } else {
+      streak = matched && __builtin_amdgcn_readlane(p, k) == i - 1 ? streak + 1 : 0;
+      run = streak >= 2;
+    }
     if (kind != 2) {
       if (lane == 0) log[acj - 1] = kind == 1 ? __builtin_amdgcn_readlane(p, k) : -1;
       acj--;
@@ -2031,10 +2142,23 @@ __device__ __forceinline__ void poa_rows(const KernelArgs& a, Arena& A, WinShare
   // The backtrack's move log goes to LDS (the ring's rows are dead now) when the query fits: a
   // global store there would make the walk's next loads wait for it as well.
   int* log = m <= 2 * RING * WR ? reinterpret_cast<int*>(&R) : A.res_from;  // H, D first
+  // Each row's heaviest in-window predecessor, for the backtrack's chain runs, after the log in
+  // LDS when both fit there (every thread, one row each).
+  int* hp = nullptr;
+  {
+    const int at = (m + 63) / 64 * 64;
+    if (log != A.res_from && at + n <= 2 * RING * WR) hp = reinterpret_cast<int*>(&R) + at;
+  }
+  if (hp && !(a.forms & MPOA_FORM_BT_RANKS)) {
+    for (int r = tid; r < n; r += NT) hp[r] = heaviest_pred(A, beg, r);
+    __syncthreads();
+  } else {
+    hp = nullptr;
+  }
   if (__builtin_amdgcn_readfirstlane(tid >> 6) == 0) {  // wave 0 (a wave-uniform branch)
     const int rc = S.narrow_bad ? MPOA_E_OVERFLOW
                    : (a.forms & MPOA_FORM_BT_LANE) ? 1
-                                                   : backtrack_wave(A, S, sc, X, log);
+                                                   : backtrack_wave(A, S, sc, X, log, hp);
     if (tid == 0) {
       // A row with more than 63 predecessors: the one-lane backtrack, which writes the
       // res_t entries itself.

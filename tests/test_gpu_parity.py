@@ -149,11 +149,11 @@ def test_held_text_follow_up_calls(ctx):
 
 
 @pytest.mark.parametrize("forms", [api.FORM_SORT_HBM, api.FORM_SORT_LANE, api.FORM_BT_LANE,
-                                   api.FORM_ADD_LANE, 15])
+                                   api.FORM_ADD_LANE, api.FORM_BT_RANKS, 15])
 def test_fallback_forms_match_oracle(ctx, forms):
     """Every read through the forms taken otherwise only for unusual graphs: the sort over a CSR
     in HBM with atomic degrees, the one-lane sort, backtrack and add_path (graph.cpp:59-123,
-    align.cpp:1572-1736, graph.cpp:170-235)."""
+    align.cpp:1572-1736, graph.cpp:170-235), and the backtrack's runs along consecutive ranks."""
     b = synth.config3(seed=25, n_windows=24)
     for result in (0, 1):
         p = api.Params(result=result)
@@ -175,7 +175,7 @@ def test_fallback_forms_reject_unknown_bits(ctx):
     b = synth.config3(seed=26, n_windows=2)
     bat = api.Batch(ctx, api.Params(), b.win_off, b.seq, b.seq_off)
     with pytest.raises(api.MpoaError):
-        bat.set_forms(128)
+        bat.set_forms(256)
     bat.free()
 
 
