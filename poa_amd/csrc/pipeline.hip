@@ -1573,7 +1573,7 @@ __device__ __forceinline__ int bt_chain(const Arena& A, const WinShared& S, cons
 
 // hp: the heaviest in-window predecessor of every row (heaviest_pred) in LDS, or null; with
 // it, runs follow the heaviest-predecessor chain (bt_chain) and are tried after every match
-// move, else they follow consecutive ranks (bt_run).
+// or substitution move, else they follow consecutive ranks (bt_run).
 template <class T>
 __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, const DpScores& sc,
                                               const DpMats<T>& X, int* log, const int* hp) {
@@ -1588,8 +1588,8 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
   int p = -1, pw = 0, pbase = 0, prule = 0, pbs = 0, pbe = 0, poff = 0;
   RowRec prr{};
   bool setup = true;
-  // Runs of match moves are tried along the heaviest-predecessor chain after every match move
-  // when hp is given (about 88% of the moves of these graphs lie in such runs, 9 moves long on
+  // Runs of match moves are tried along the heaviest-predecessor chain after every match or
+  // substitution move when hp is given (about 88% of the moves of these graphs lie in such runs, 9 moves long on
   // average), else along consecutive ranks after two match moves in a row to the previous rank
   // (where a run is likely to go on); and again after a run that used its whole width. The
   // width starts at 8 moves and doubles after such a run (up to 63), since a run's loads cost
@@ -1723,7 +1723,7 @@ __device__ __forceinline__ int backtrack_wave(const Arena& A, WinShared& S, cons
     }
     if (kind == 0) return MPOA_E_BACKTRACK;
     if (hp) {
-      run = matched;  // most match moves of these graphs start or continue a chain run
+      run = kind == 1;  // after a match or a substitution a chain run most likely follows
     } else {
       streak = matched && __builtin_amdgcn_readlane(p, k) == i - 1 ? streak + 1 : 0;
       run = streak >= 2;
